@@ -1,0 +1,1095 @@
+// CPU oracle for the explicit elastic forward / OtD adjoint / gradient path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, on the CPU, the reference
+// algorithm the HIP path must reproduce.  Only the tests, the smoke check and
+// the CPU-baseline leg of the benchmark load it; no product code links it.
+//
+// Restated from (paths relative to step_4_3D_Forward_and_Inverse/):
+//   SHAPE_3_27            Shape_Functions.F90:124-274
+//   GAUSS / GAUSS_Lobatto Source1.F90:6-134 (3-point rule = GLL, Source1.F90:27-32)
+//   PML_FINDER_3D         PML.F90:325-891 (degree 2, ratio 0)
+//   ELEMENT_K_3D          Element_Matrices_3D.F90:7-220
+//   ELEMENT_M_3D          Element_Matrices_3D.F90:228-415
+//   ELEMENT_EN_3D         Element_Matrices_3D.F90:423-603
+//   ELEMENT_EA_3D         Element_Matrices_3D.F90:618-882
+//   ELEMENT_FS_3D         Element_Matrices_3D.F90:890-1096
+//   ELEMENT_g_3D          Element_Matrices_3D.F90:1725-1890
+//   assembly              Assemble_PETSc_3D.F90:6-184 (RD), 192-436 (PML),
+//                         Solve_Forward.F90:63-97 (assembly end, M^-1)
+//   EXPLICIT_RK4_PETSC_PML_3D              Solver_PETSc_Explicit.F90:660-1117
+//   EXPLICIT_RK4_PETSC_PML_3D_Adjoint_OtD  Solver_PETSc_Explicit.F90:1125-1405
+//   EXPLICIT_RK4_PETSC_PML_3D_Control_OtD  Solver_PETSc_Explicit.F90:1413-1522
+//   Update_g              Assemble_PETSc_3D.F90:556-661
+//   Compute_Misfit        Solver_PETSc_Inverse.F90:591-650
+//   TIME_FUNCTION         IO.F90:541-733
+//
+// PETSc 3.4 semantics restated from its documentation and published source
+// (not vendored in the reference): MatSetValuesLocal is row-oriented with
+// ADD_VALUES, negative indices are dropped, zero values are ignored
+// (MAT_IGNORE_ZERO_ENTRIES, PETSc_Global_Objects.F90:61-66); SeqAIJ
+// MatMultAdd computes z_i = y_i + sum_j a_ij x_j in column order;
+// MatMultTransposeAdd scatters row by row; MatDiagonalScale scales rows then
+// columns; VecWAXPY w = y + a x; VecMAXPY x += a0 y0 + a1 y1 + ...
+//
+// Compiled with -O2 -ffp-contract=off (no fused multiply-add) so that the
+// operation order of the Fortran expressions is kept.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <vector>
+#include <algorithm>
+#include <numeric>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef int64_t i64;
+
+namespace {
+
+const double GLL_X[3] = {-1.0, 0.0, 1.0};
+// WINT(1:3,3) of GAUSS / GAUSS_Lobatto (Source1.F90:29-31)
+const double GLL_W[3] = {0.33333333333333333333, 1.33333333333333333333, 0.33333333333333333333};
+
+// ---------------------------------------------------------------- SHAPE_3_27
+void shape_3_27(double X1, double X2, double X3, double* FN, double* DFXI) {
+  // FN(27), DFXI(27,3) column-major: DFXI[n + 27*k]
+#define D(n, k) DFXI[(n - 1) + 27 * (k - 1)]
+  FN[0] = 0.125 * X1 * -X2 * -X3 * (1.0 + X1) * (1.0 - X2) * (1.0 - X3);
+  FN[1] = 0.125 * X1 * X2 * -X3 * (1.0 + X1) * (1.0 + X2) * (1.0 - X3);
+  FN[2] = 0.125 * -X1 * X2 * -X3 * (1.0 - X1) * (1.0 + X2) * (1.0 - X3);
+  FN[3] = 0.125 * -X1 * -X2 * -X3 * (1.0 - X1) * (1.0 - X2) * (1.0 - X3);
+  FN[4] = 0.125 * X1 * -X2 * X3 * (1.0 + X1) * (1.0 - X2) * (1.0 + X3);
+  FN[5] = 0.125 * X1 * X2 * X3 * (1.0 + X1) * (1.0 + X2) * (1.0 + X3);
+  FN[6] = 0.125 * -X1 * X2 * X3 * (1.0 - X1) * (1.0 + X2) * (1.0 + X3);
+  FN[7] = 0.125 * -X1 * -X2 * X3 * (1.0 - X1) * (1.0 - X2) * (1.0 + X3);
+  const double X1s = X1 * X1, X2s = X2 * X2, X3s = X3 * X3;  // X**2 (integer power)
+  FN[8] = 0.25 * X1 * -X3 * (1.0 + X1) * (1.0 - X3) * (1.0 - X2s);
+  FN[10] = 0.25 * -X1 * -X3 * (1.0 - X1) * (1.0 - X3) * (1.0 - X2s);
+  FN[16] = 0.25 * X1 * X3 * (1.0 + X1) * (1.0 + X3) * (1.0 - X2s);
+  FN[18] = 0.25 * -X1 * X3 * (1.0 - X1) * (1.0 + X3) * (1.0 - X2s);
+  FN[9] = 0.25 * X2 * -X3 * (1.0 + X2) * (1.0 - X3) * (1.0 - X1s);
+  FN[11] = 0.25 * -X2 * -X3 * (1.0 - X2) * (1.0 - X3) * (1.0 - X1s);
+  FN[17] = 0.25 * X2 * X3 * (1.0 + X2) * (1.0 + X3) * (1.0 - X1s);
+  FN[19] = 0.25 * -X2 * X3 * (1.0 - X2) * (1.0 + X3) * (1.0 - X1s);
+  FN[12] = 0.25 * -X2 * X1 * (1.0 - X2) * (1.0 + X1) * (1.0 - X3s);
+  FN[13] = 0.25 * X2 * X1 * (1.0 + X2) * (1.0 + X1) * (1.0 - X3s);
+  FN[14] = 0.25 * X2 * -X1 * (1.0 + X2) * (1.0 - X1) * (1.0 - X3s);
+  FN[15] = 0.25 * -X2 * -X1 * (1.0 - X2) * (1.0 - X1) * (1.0 - X3s);
+  FN[20] = 0.50 * -X3 * (1.0 - X3) * (1.0 - X1s) * (1.0 - X2s);
+  FN[25] = 0.50 * X3 * (1.0 + X3) * (1.0 - X1s) * (1.0 - X2s);
+  FN[21] = 0.50 * X1 * (1.0 + X1) * (1.0 - X2s) * (1.0 - X3s);
+  FN[23] = 0.50 * -X1 * (1.0 - X1) * (1.0 - X2s) * (1.0 - X3s);
+  FN[22] = 0.50 * X2 * (1.0 + X2) * (1.0 - X1s) * (1.0 - X3s);
+  FN[24] = 0.50 * -X2 * (1.0 - X2) * (1.0 - X1s) * (1.0 - X3s);
+  FN[26] = (1.0 - X1s) * (1.0 - X2s) * (1.0 - X3s);
+
+  D(1, 1) = 0.125 * -X2 * -X3 * (1.0 - X2) * (1.0 - X3) * (1.0 + 2.0 * X1);
+  D(2, 1) = 0.125 * X2 * -X3 * (1.0 + X2) * (1.0 - X3) * (1.0 + 2.0 * X1);
+  D(3, 1) = 0.125 * X2 * -X3 * (1.0 + X2) * (1.0 - X3) * (-1.0 + 2.0 * X1);
+  D(4, 1) = 0.125 * -X2 * -X3 * (1.0 - X2) * (1.0 - X3) * (-1.0 + 2.0 * X1);
+  D(5, 1) = 0.125 * -X2 * X3 * (1.0 - X2) * (1.0 + X3) * (1.0 + 2.0 * X1);
+  D(6, 1) = 0.125 * X2 * X3 * (1.0 + X2) * (1.0 + X3) * (1.0 + 2.0 * X1);
+  D(7, 1) = 0.125 * X2 * X3 * (1.0 + X2) * (1.0 + X3) * (-1.0 + 2.0 * X1);
+  D(8, 1) = 0.125 * -X2 * X3 * (1.0 - X2) * (1.0 + X3) * (-1.0 + 2.0 * X1);
+  D(9, 1) = 0.25 * -X3 * (1.0 - X3) * (1.0 - X2s) * (1.0 + 2.0 * X1);
+  D(11, 1) = 0.25 * -X3 * (1.0 - X3) * (1.0 - X2s) * (-1.0 + 2.0 * X1);
+  D(17, 1) = 0.25 * X3 * (1.0 + X3) * (1.0 - X2s) * (1.0 + 2.0 * X1);
+  D(19, 1) = 0.25 * X3 * (1.0 + X3) * (1.0 - X2s) * (-1.0 + 2.0 * X1);
+  D(10, 1) = 0.50 * X2 * -X3 * (1.0 + X2) * (1.0 - X3) * (-X1);
+  D(12, 1) = 0.50 * -X2 * -X3 * (1.0 - X2) * (1.0 - X3) * (-X1);
+  D(18, 1) = 0.50 * X2 * X3 * (1.0 + X2) * (1.0 + X3) * (-X1);
+  D(20, 1) = 0.50 * -X2 * X3 * (1.0 - X2) * (1.0 + X3) * (-X1);
+  D(13, 1) = 0.25 * -X2 * (1.0 - X2) * (1.0 - X3s) * (1.0 + 2.0 * X1);
+  D(14, 1) = 0.25 * X2 * (1.0 + X2) * (1.0 - X3s) * (1.0 + 2.0 * X1);
+  D(15, 1) = 0.25 * X2 * (1.0 + X2) * (1.0 - X3s) * (-1.0 + 2.0 * X1);
+  D(16, 1) = 0.25 * -X2 * (1.0 - X2) * (1.0 - X3s) * (-1.0 + 2.0 * X1);
+  D(21, 1) = -X3 * (1.0 - X3) * (-X1) * (1.0 - X2s);
+  D(26, 1) = X3 * (1.0 + X3) * (-X1) * (1.0 - X2s);
+  D(22, 1) = 0.50 * (1.0 - X2s) * (1.0 - X3s) * (1.0 + 2.0 * X1);
+  D(24, 1) = 0.50 * (1.0 - X2s) * (1.0 - X3s) * (-1.0 + 2.0 * X1);
+  D(23, 1) = X2 * (1.0 + X2) * (-X1) * (1.0 - X3s);
+  D(25, 1) = -X2 * (1.0 - X2) * (-X1) * (1.0 - X3s);
+  D(27, 1) = (-2.0 * X1) * (1.0 - X2s) * (1.0 - X3s);
+
+  D(1, 2) = 0.125 * X1 * -X3 * (1.0 + X1) * (-1.0 + 2.0 * X2) * (1.0 - X3);
+  D(2, 2) = 0.125 * X1 * -X3 * (1.0 + X1) * (+1.0 + 2.0 * X2) * (1.0 - X3);
+  D(3, 2) = 0.125 * -X1 * -X3 * (1.0 - X1) * (+1.0 + 2.0 * X2) * (1.0 - X3);
+  D(4, 2) = 0.125 * -X1 * -X3 * (1.0 - X1) * (-1.0 + 2.0 * X2) * (1.0 - X3);
+  D(5, 2) = 0.125 * X1 * X3 * (1.0 + X1) * (-1.0 + 2.0 * X2) * (1.0 + X3);
+  D(6, 2) = 0.125 * X1 * X3 * (1.0 + X1) * (+1.0 + 2.0 * X2) * (1.0 + X3);
+  D(7, 2) = 0.125 * -X1 * X3 * (1.0 - X1) * (+1.0 + 2.0 * X2) * (1.0 + X3);
+  D(8, 2) = 0.125 * -X1 * X3 * (1.0 - X1) * (-1.0 + 2.0 * X2) * (1.0 + X3);
+  D(9, 2) = 0.50 * X1 * -X3 * (1.0 + X1) * (1.0 - X3) * (-X2);
+  D(11, 2) = 0.50 * -X1 * -X3 * (1.0 - X1) * (1.0 - X3) * (-X2);
+  D(17, 2) = 0.50 * X1 * X3 * (1.0 + X1) * (1.0 + X3) * (-X2);
+  D(19, 2) = 0.50 * -X1 * X3 * (1.0 - X1) * (1.0 + X3) * (-X2);
+  D(10, 2) = 0.25 * -X3 * (+1.0 + 2.0 * X2) * (1.0 - X3) * (1.0 - X1s);
+  D(12, 2) = 0.25 * -X3 * (-1.0 + 2.0 * X2) * (1.0 - X3) * (1.0 - X1s);
+  D(18, 2) = 0.25 * X3 * (+1.0 + 2.0 * X2) * (1.0 + X3) * (1.0 - X1s);
+  D(20, 2) = 0.25 * X3 * (-1.0 + 2.0 * X2) * (1.0 + X3) * (1.0 - X1s);
+  D(13, 2) = 0.25 * X1 * (-1.0 + 2.0 * X2) * (1.0 + X1) * (1.0 - X3s);
+  D(14, 2) = 0.25 * X1 * (+1.0 + 2.0 * X2) * (1.0 + X1) * (1.0 - X3s);
+  D(15, 2) = 0.25 * -X1 * (+1.0 + 2.0 * X2) * (1.0 - X1) * (1.0 - X3s);
+  D(16, 2) = 0.25 * -X1 * (-1.0 + 2.0 * X2) * (1.0 - X1) * (1.0 - X3s);
+  D(21, 2) = -X3 * (1.0 - X3) * (1.0 - X1s) * (-X2);
+  D(26, 2) = X3 * (1.0 + X3) * (1.0 - X1s) * (-X2);
+  D(22, 2) = X1 * (1.0 + X1) * (-X2) * (1.0 - X3s);
+  D(24, 2) = -X1 * (1.0 - X1) * (-X2) * (1.0 - X3s);
+  D(23, 2) = 0.50 * (+1.0 + 2.0 * X2) * (1.0 - X1s) * (1.0 - X3s);
+  D(25, 2) = 0.50 * (-1.0 + 2.0 * X2) * (1.0 - X1s) * (1.0 - X3s);
+  D(27, 2) = (1.0 - X1s) * (-2.0 * X2) * (1.0 - X3s);
+
+  D(1, 3) = 0.125 * X1 * -X2 * (1.0 + X1) * (1.0 - X2) * (-1.0 + 2.0 * X3);
+  D(2, 3) = 0.125 * X1 * X2 * (1.0 + X1) * (1.0 + X2) * (-1.0 + 2.0 * X3);
+  D(3, 3) = 0.125 * -X1 * X2 * (1.0 - X1) * (1.0 + X2) * (-1.0 + 2.0 * X3);
+  D(4, 3) = 0.125 * -X1 * -X2 * (1.0 - X1) * (1.0 - X2) * (-1.0 + 2.0 * X3);
+  D(5, 3) = 0.125 * X1 * -X2 * (1.0 + X1) * (1.0 - X2) * (+1.0 + 2.0 * X3);
+  D(6, 3) = 0.125 * X1 * X2 * (1.0 + X1) * (1.0 + X2) * (+1.0 + 2.0 * X3);
+  D(7, 3) = 0.125 * -X1 * X2 * (1.0 - X1) * (1.0 + X2) * (+1.0 + 2.0 * X3);
+  D(8, 3) = 0.125 * -X1 * -X2 * (1.0 - X1) * (1.0 - X2) * (+1.0 + 2.0 * X3);
+  D(9, 3) = 0.25 * X1 * (1.0 + X1) * (-1.0 + 2.0 * X3) * (1.0 - X2s);
+  D(11, 3) = 0.25 * -X1 * (1.0 - X1) * (-1.0 + 2.0 * X3) * (1.0 - X2s);
+  D(17, 3) = 0.25 * X1 * (1.0 + X1) * (+1.0 + 2.0 * X3) * (1.0 - X2s);
+  D(19, 3) = 0.25 * -X1 * (1.0 - X1) * (+1.0 + 2.0 * X3) * (1.0 - X2s);
+  D(10, 3) = 0.25 * X2 * (1.0 + X2) * (-1.0 + 2.0 * X3) * (1.0 - X1s);
+  D(12, 3) = 0.25 * -X2 * (1.0 - X2) * (-1.0 + 2.0 * X3) * (1.0 - X1s);
+  D(18, 3) = 0.25 * X2 * (1.0 + X2) * (+1.0 + 2.0 * X3) * (1.0 - X1s);
+  D(20, 3) = 0.25 * -X2 * (1.0 - X2) * (+1.0 + 2.0 * X3) * (1.0 - X1s);
+  D(13, 3) = 0.50 * -X2 * X1 * (1.0 - X2) * (1.0 + X1) * (-X3);
+  D(14, 3) = 0.50 * X2 * X1 * (1.0 + X2) * (1.0 + X1) * (-X3);
+  D(15, 3) = 0.50 * X2 * -X1 * (1.0 + X2) * (1.0 - X1) * (-X3);
+  D(16, 3) = 0.50 * -X2 * -X1 * (1.0 - X2) * (1.0 - X1) * (-X3);
+  D(21, 3) = 0.50 * (-1.0 + 2.0 * X3) * (1.0 - X1s) * (1.0 - X2s);
+  D(26, 3) = 0.50 * (+1.0 + 2.0 * X3) * (1.0 - X1s) * (1.0 - X2s);
+  D(22, 3) = X1 * (1.0 + X1) * (1.0 - X2s) * (-X3);
+  D(24, 3) = -X1 * (1.0 - X1) * (1.0 - X2s) * (-X3);
+  D(23, 3) = X2 * (1.0 + X2) * (1.0 - X1s) * (-X3);
+  D(25, 3) = -X2 * (1.0 - X2) * (1.0 - X1s) * (-X3);
+  D(27, 3) = (1.0 - X1s) * (1.0 - X2s) * (-2.0 * X3);
+#undef D
+}
+
+// ------------------------------------------------------------ PML_FINDER_3D
+// pml_param: PML_PARAM(6,4) column-major, P(f,c) = p[(f-1) + 6*(c-1)]
+void pml_finder_3d(const double* XG, const double* p, double* PAB) {
+  auto P = [&](int f, int c) { return p[(f - 1) + 6 * (c - 1)]; };
+  for (int i = 0; i < 19; ++i) PAB[i] = 0.0;
+  PAB[0] = 1.0; PAB[1] = 1.0; PAB[2] = 1.0;
+  const double deg = 2.0, ratio = 0.0;
+  double dxa_y = 0, dxa_z = 0, dxb_y = 0, dxb_z = 0;
+  double dya_x = 0, dya_z = 0, dyb_x = 0, dyb_z = 0;
+  double dza_x = 0, dza_y = 0, dzb_x = 0, dzb_y = 0;
+  auto pw = [&](double f) { return std::pow(f, deg); };
+  auto pw1 = [&](double f) { return std::pow(f, deg - 1.0); };
+  // x zone
+  {
+    bool in = false; double diff = 0, vn = 0, a0 = 0, b0 = 0, L = 1;
+    if (XG[0] >= P(1, 4)) { in = true; diff = XG[0] - P(1, 4); vn = 1.0; a0 = P(1, 1); b0 = P(1, 2); L = P(1, 3); }
+    else if (XG[0] <= P(2, 4)) { in = true; diff = XG[0] - P(2, 4); vn = -1.0; a0 = P(2, 1); b0 = P(2, 2); L = P(2, 3); }
+    if (in) {
+      double F1 = diff * vn / L, F4 = vn / L;
+      PAB[0] = PAB[0] + a0 * pw(F1);
+      PAB[3] = PAB[3] + b0 * pw(F1);
+      PAB[1] = PAB[1] + (a0 * pw(F1)) * ratio;
+      PAB[2] = PAB[2] + (a0 * pw(F1)) * ratio;
+      PAB[4] = PAB[4] + (b0 * pw(F1)) * ratio;
+      PAB[5] = PAB[5] + (b0 * pw(F1)) * ratio;
+      dxa_y = (a0 * deg * F4 * pw1(F1)) * ratio; dxa_z = (a0 * deg * F4 * pw1(F1)) * ratio;
+      dxb_y = (b0 * deg * F4 * pw1(F1)) * ratio; dxb_z = (b0 * deg * F4 * pw1(F1)) * ratio;
+    }
+  }
+  // y zone
+  {
+    bool in = false; double diff = 0, vn = 0, a0 = 0, b0 = 0, L = 1;
+    if (XG[1] >= P(3, 4)) { in = true; diff = XG[1] - P(3, 4); vn = 1.0; a0 = P(3, 1); b0 = P(3, 2); L = P(3, 3); }
+    else if (XG[1] <= P(4, 4)) { in = true; diff = XG[1] - P(4, 4); vn = -1.0; a0 = P(4, 1); b0 = P(4, 2); L = P(4, 3); }
+    if (in) {
+      double F2 = diff * vn / L, F5 = vn / L;
+      PAB[1] = PAB[1] + a0 * pw(F2);
+      PAB[4] = PAB[4] + b0 * pw(F2);
+      PAB[0] = PAB[0] + (a0 * pw(F2)) * ratio;
+      PAB[2] = PAB[2] + (a0 * pw(F2)) * ratio;
+      PAB[3] = PAB[3] + (b0 * pw(F2)) * ratio;
+      PAB[5] = PAB[5] + (b0 * pw(F2)) * ratio;
+      dya_x = (a0 * deg * F5 * pw1(F2)) * ratio; dya_z = (a0 * deg * F5 * pw1(F2)) * ratio;
+      dyb_x = (b0 * deg * F5 * pw1(F2)) * ratio; dyb_z = (b0 * deg * F5 * pw1(F2)) * ratio;
+    }
+  }
+  // z zone (top zone is an error in the reference: PML.F90:629-633)
+  {
+    if (XG[2] > P(5, 4)) {
+      std::fprintf(stderr, "oracle: top PML zone detected\n");
+    } else if (XG[2] <= P(6, 4)) {
+      double diff = XG[2] - P(6, 4), vn = -1.0, a0 = P(6, 1), b0 = P(6, 2), L = P(6, 3);
+      double F3 = diff * vn / L, F6 = vn / L;
+      PAB[2] = PAB[2] + a0 * pw(F3);
+      PAB[5] = PAB[5] + b0 * pw(F3);
+      PAB[0] = PAB[0] + (a0 * pw(F3)) * ratio;
+      PAB[1] = PAB[1] + (a0 * pw(F3)) * ratio;
+      PAB[3] = PAB[3] + (b0 * pw(F3)) * ratio;
+      PAB[4] = PAB[4] + (b0 * pw(F3)) * ratio;
+      dza_x = (a0 * deg * F6 * pw1(F3)) * ratio; dza_y = (a0 * deg * F6 * pw1(F3)) * ratio;
+      dzb_x = (b0 * deg * F6 * pw1(F3)) * ratio; dzb_y = (b0 * deg * F6 * pw1(F3)) * ratio;
+    }
+  }
+  const double* A = PAB;
+  PAB[6] = A[0] * A[1] * A[2];
+  PAB[7] = A[0] * A[1] * A[5] + A[0] * A[2] * A[4] + A[1] * A[2] * A[3];
+  PAB[8] = A[0] * A[4] * A[5] + A[1] * A[3] * A[5] + A[2] * A[3] * A[4];
+  PAB[9] = A[3] * A[4] * A[5];
+  PAB[10] = dza_x * A[1] + dza_y * A[0];
+  PAB[11] = dya_x * A[2] + dya_z * A[0];
+  PAB[12] = dxa_y * A[2] + dxa_z * A[1];
+  PAB[13] = dza_x * A[4] + A[0] * dzb_y + dza_y * A[3] + A[1] * dzb_x;
+  PAB[14] = dya_x * A[5] + A[0] * dyb_z + dya_z * A[3] + A[2] * dyb_x;
+  PAB[15] = dxa_y * A[5] + A[1] * dxb_z + dxa_z * A[4] + A[2] * dxb_y;
+  PAB[16] = dzb_x * A[4] + dzb_y * A[3];
+  PAB[17] = dyb_x * A[5] + dyb_z * A[3];
+  PAB[18] = dxb_y * A[5] + dxb_z * A[4];
+}
+
+// Geometry at one quadrature point: Jacobian from XT(3,27), DFX(27,3), FAC0.
+struct QPoint {
+  double FN[27], DFXI[81], DFX[81], detj, fac0, XG[3];
+};
+
+void qpoint(const double* XT, int lx, int ly, int lz, QPoint& q, bool need_dfx) {
+  const double X1 = GLL_X[lx], X2 = GLL_X[ly], X3 = GLL_X[lz];
+  const double WSTAR = GLL_W[lx] * GLL_W[ly] * GLL_W[lz];
+  shape_3_27(X1, X2, X3, q.FN, q.DFXI);
+  double DJ[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int n = 0; n < 27; ++n) s += XT[i + 3 * n] * q.DFXI[n + 27 * j];
+      DJ[i][j] = s;
+    }
+  // DJ(i,j) (1-based) = DJ[i-1][j-1]
+#define J(a, b) DJ[a - 1][b - 1]
+  const double DETJ = J(1, 1) * J(2, 2) * J(3, 3) + J(2, 1) * J(3, 2) * J(1, 3) + J(3, 1) * J(1, 2) * J(2, 3) -
+                      J(1, 3) * J(2, 2) * J(3, 1) - J(2, 3) * J(3, 2) * J(1, 1) - J(3, 3) * J(1, 2) * J(2, 1);
+  if (DETJ <= 0) std::fprintf(stderr, "oracle: |J|<0 NEGATIVE DET. JACOBIAN\n");
+  q.detj = DETJ;
+  q.fac0 = WSTAR * DETJ;
+  if (need_dfx) {
+    double DJI[3][3];
+    DJI[0][0] = +J(3, 3) * J(2, 2) - J(3, 2) * J(2, 3);
+    DJI[0][1] = -J(3, 3) * J(1, 2) + J(3, 2) * J(1, 3);
+    DJI[0][2] = +J(2, 3) * J(1, 2) - J(2, 2) * J(1, 3);
+    DJI[1][0] = -J(3, 3) * J(2, 1) + J(3, 1) * J(2, 3);
+    DJI[1][1] = +J(3, 3) * J(1, 1) - J(3, 1) * J(1, 3);
+    DJI[1][2] = -J(2, 3) * J(1, 1) + J(2, 1) * J(1, 3);
+    DJI[2][0] = +J(3, 2) * J(2, 1) - J(3, 1) * J(2, 2);
+    DJI[2][1] = -J(3, 2) * J(1, 1) + J(3, 1) * J(1, 2);
+    DJI[2][2] = +J(2, 2) * J(1, 1) - J(2, 1) * J(1, 2);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) DJI[a][b] = DJI[a][b] / DETJ;
+    for (int n = 0; n < 27; ++n)
+      for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
+        for (int m = 0; m < 3; ++m) s += q.DFXI[n + 27 * m] * DJI[m][k];
+        q.DFX[n + 27 * k] = s;
+      }
+  }
+#undef J
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0;
+    for (int n = 0; n < 27; ++n) s += XT[i + 3 * n] * q.FN[n];
+    q.XG[i] = s;
+  }
+}
+
+double dot27(const double* a, const double* b) {
+  double s = 0.0;
+  for (int n = 0; n < 27; ++n) s += a[n] * b[n];
+  return s;
+}
+
+// ------------------------------------------------------------- ELEMENT_K_3D
+void element_k(const double* XT, const double* lam, const double* mu, double* EK) {
+  std::memset(EK, 0, sizeof(double) * 81 * 81);
+  QPoint q;
+  for (int lz = 0; lz < 3; ++lz)
+    for (int ly = 0; ly < 3; ++ly)
+      for (int lx = 0; lx < 3; ++lx) {
+        qpoint(XT, lx, ly, lz, q, true);
+        const double zl = dot27(q.FN, lam), zm = dot27(q.FN, mu);
+        const double FAC1 = zl + 2.0 * zm, FAC2 = zl, FAC3 = zm;
+        const double* DFX = q.DFX;
+        // rows whose DFX vanishes at this point contribute exact zeros
+        int live[27], nl = 0;
+        for (int n = 0; n < 27; ++n)
+          if (DFX[n] != 0.0 || DFX[n + 27] != 0.0 || DFX[n + 54] != 0.0) live[nl++] = n;
+        for (int a = 0; a < nl; ++a) {
+          const int I = live[a];
+          for (int b = 0; b < nl; ++b) {
+            const int Jn = live[b];
+            const double H1 = DFX[I] * DFX[Jn], H2 = DFX[I + 27] * DFX[Jn + 27], H3 = DFX[I + 54] * DFX[Jn + 54];
+            const double H4 = DFX[I] * DFX[Jn + 27], H5 = DFX[I + 27] * DFX[Jn];
+            const double H6 = DFX[I] * DFX[Jn + 54], H7 = DFX[I + 54] * DFX[Jn];
+            const double H8 = DFX[I + 27] * DFX[Jn + 54], H9 = DFX[I + 54] * DFX[Jn + 27];
+            const double G1 = FAC1 * H1 + FAC3 * (H2 + H3);
+            const double G2 = FAC2 * H4 + FAC3 * H5;
+            const double G3 = FAC2 * H6 + FAC3 * H7;
+            const double G4 = FAC2 * H5 + FAC3 * H4;
+            const double G5 = FAC1 * H2 + FAC3 * (H1 + H3);
+            const double G6 = FAC2 * H8 + FAC3 * H9;
+            const double G7 = FAC2 * H7 + FAC3 * H6;
+            const double G8 = FAC2 * H9 + FAC3 * H8;
+            const double G9 = FAC1 * H3 + FAC3 * (H1 + H2);
+            const int I1 = I, I2 = I + 27, I3 = I + 54, J1 = Jn, J2 = Jn + 27, J3 = Jn + 54;
+            const double F0 = q.fac0;
+            EK[I1 + 81 * J1] += G1 * F0; EK[I1 + 81 * J2] += G2 * F0; EK[I1 + 81 * J3] += G3 * F0;
+            EK[I2 + 81 * J1] += G4 * F0; EK[I2 + 81 * J2] += G5 * F0; EK[I2 + 81 * J3] += G6 * F0;
+            EK[I3 + 81 * J1] += G7 * F0; EK[I3 + 81 * J2] += G8 * F0; EK[I3 + 81 * J3] += G9 * F0;
+          }
+        }
+      }
+}
+
+// diagonal factor of ELEMENT_M_3D at one point
+double m_factor(int iflag, double RO, const double* PAB) {
+  switch (iflag) {
+    case 201: return RO;
+    case 202: return RO * PAB[6];
+    case 203: return RO * PAB[7];
+    case 204: return RO * PAB[8];
+    case 205: return RO * PAB[9];
+  }
+  return 0.0;
+}
+
+// ------------------------------------------------------------- ELEMENT_M_3D
+// Full 81x81 matrix (FN is a Kronecker delta at the GLL points, so only the
+// (q,q) entries receive a non-zero product).
+void element_m(const double* XT, double RO, const double* PP, int iflag, double* EM) {
+  std::memset(EM, 0, sizeof(double) * 81 * 81);
+  QPoint q;
+  double PAB[19];
+  for (int lz = 0; lz < 3; ++lz)
+    for (int ly = 0; ly < 3; ++ly)
+      for (int lx = 0; lx < 3; ++lx) {
+        qpoint(XT, lx, ly, lz, q, false);
+        if (iflag != 201) pml_finder_3d(q.XG, PP, PAB);
+        const double FAC1 = m_factor(iflag, RO, PAB);
+        for (int I = 0; I < 27; ++I) {
+          if (q.FN[I] == 0.0) continue;
+          for (int Jn = 0; Jn < 27; ++Jn) {
+            if (q.FN[Jn] == 0.0) continue;
+            const double G1 = FAC1 * (q.FN[I] * q.FN[Jn]);
+            for (int c = 0; c < 3; ++c) EM[(I + 27 * c) + 81 * (Jn + 27 * c)] += G1 * q.fac0;
+          }
+        }
+      }
+}
+
+// ------------------------------------------------------------ ELEMENT_EN_3D
+void element_en(const double* XT, const double* PP, int iflag, double* EN) {
+  std::memset(EN, 0, sizeof(double) * 162 * 162);
+  QPoint q;
+  double PAB[19];
+  const double FAC1 = 2.0;
+  for (int lz = 0; lz < 3; ++lz)
+    for (int ly = 0; ly < 3; ++ly)
+      for (int lx = 0; lx < 3; ++lx) {
+        qpoint(XT, lx, ly, lz, q, false);
+        pml_finder_3d(q.XG, PP, PAB);
+        const double FAC2 = PAB[6 + (iflag - 301)];
+        for (int I = 0; I < 27; ++I) {
+          if (q.FN[I] == 0.0) continue;
+          for (int Jn = 0; Jn < 27; ++Jn) {
+            if (q.FN[Jn] == 0.0) continue;
+            const double H1 = q.FN[I] * q.FN[Jn];
+            const double G1 = FAC2 * H1, G2 = FAC1 * FAC2 * H1;
+            for (int c = 0; c < 3; ++c) EN[(I + 27 * c) + 162 * (Jn + 27 * c)] += G1 * q.fac0;
+            for (int c = 3; c < 6; ++c) EN[(I + 27 * c) + 162 * (Jn + 27 * c)] += G2 * q.fac0;
+          }
+        }
+      }
+}
+
+// ------------------------------------------------------------ ELEMENT_EA_3D
+// EA(81,162): only J = q carries a non-zero FN(J), so the inner loop runs over
+// the rows I for the single column node J = q (exact: the skipped terms are
+// signed zeros).
+void element_ea(const double* XT, const double* lam, const double* mu, const double* PP, int iflag,
+                double* EA) {
+  std::memset(EA, 0, sizeof(double) * 81 * 162);
+  double FAC7, FAC8, FAC9, FAC10;
+  const bool lower = iflag >= 104;
+  if (!lower) { FAC7 = 1.0; FAC8 = 0.0; FAC9 = 1.0; FAC10 = 1.0; }
+  else { FAC7 = FAC8 = FAC9 = 0.0; FAC10 = 0.0; }
+  QPoint q;
+  double PAB[19];
+  for (int lz = 0; lz < 3; ++lz)
+    for (int ly = 0; ly < 3; ++ly)
+      for (int lx = 0; lx < 3; ++lx) {
+        qpoint(XT, lx, ly, lz, q, true);
+        if (lower) {
+          const double zl = dot27(q.FN, lam), zm = dot27(q.FN, mu);
+          FAC7 = zl + 2.0 * zm; FAC8 = zl; FAC9 = 2.0 * zm;
+        }
+        pml_finder_3d(q.XG, PP, PAB);
+        double FAC1, FAC2, FAC3, FAC4, FAC5, FAC6;
+        const double* A = PAB;
+        switch (iflag) {
+          case 101: case 104:
+            FAC1 = A[0] * A[1]; FAC2 = A[0] * A[2]; FAC3 = A[1] * A[2];
+            FAC4 = A[10]; FAC5 = A[11]; FAC6 = A[12]; break;
+          case 102: case 105:
+            FAC1 = A[0] * A[4] + A[1] * A[3]; FAC2 = A[0] * A[5] + A[2] * A[3]; FAC3 = A[1] * A[5] + A[2] * A[4];
+            FAC4 = A[13]; FAC5 = A[14]; FAC6 = A[15]; break;
+          default:
+            FAC1 = A[3] * A[4]; FAC2 = A[3] * A[5]; FAC3 = A[4] * A[5];
+            FAC4 = A[16]; FAC5 = A[17]; FAC6 = A[18]; break;
+        }
+        const double* DFX = q.DFX;
+        for (int Jn = 0; Jn < 27; ++Jn) {
+          if (q.FN[Jn] == 0.0) continue;
+          for (int I = 0; I < 27; ++I) {
+            const double H1 = DFX[I] * q.FN[Jn], H2 = DFX[I + 27] * q.FN[Jn], H3 = DFX[I + 54] * q.FN[Jn];
+            const double H4 = q.FN[I] * q.FN[Jn];
+            const double G1 = FAC7 * FAC3 * H1 + FAC10 * FAC6 * H4;
+            const double G2 = FAC8 * FAC3 * H1;
+            const double G3 = FAC9 * FAC2 * H2 + FAC10 * FAC5 * H4;
+            const double G4 = FAC9 * FAC1 * H3 + FAC10 * FAC4 * H4;
+            const double G5 = FAC8 * FAC2 * H2;
+            const double G6 = FAC7 * FAC2 * H2 + FAC10 * FAC5 * H4;
+            const double G7 = FAC9 * FAC3 * H1 + FAC10 * FAC6 * H4;
+            const double G8 = FAC8 * FAC1 * H3;
+            const double G9 = FAC7 * FAC1 * H3 + FAC10 * FAC4 * H4;
+            const double F0 = q.fac0;
+            const int I1 = I, I2 = I + 27, I3 = I + 54;
+            const int J1 = Jn, J2 = Jn + 27, J3 = Jn + 54, J4 = Jn + 81, J5 = Jn + 108, J6 = Jn + 135;
+#define E(r, c) EA[(r) + 81 * (c)]
+            E(I1, J1) += G1 * F0; E(I1, J2) += G2 * F0; E(I1, J3) += G2 * F0; E(I1, J4) += G3 * F0; E(I1, J5) += G4 * F0;
+            E(I2, J1) += G5 * F0; E(I2, J2) += G6 * F0; E(I2, J3) += G5 * F0; E(I2, J4) += G7 * F0; E(I2, J6) += G4 * F0;
+            E(I3, J1) += G8 * F0; E(I3, J2) += G8 * F0; E(I3, J3) += G9 * F0; E(I3, J5) += G7 * F0; E(I3, J6) += G3 * F0;
+#undef E
+          }
+        }
+      }
+}
+
+// ------------------------------------------------------------ ELEMENT_FS_3D
+void element_fs(const double* XT, int NFACE, double* EFS) {
+  std::memset(EFS, 0, sizeof(double) * 81);
+  if (NFACE == 0) return;
+  if (NFACE == 6) NFACE = 5;
+  const int LFACE[6] = {2, 2, 3, 3, 1, 1};
+  const double FVAL[6] = {+1.0, -1.0, +1.0, -1.0, +1.0, -1.0};
+  const int IPERM[3] = {2, 3, 1};
+  const int L = LFACE[NFACE - 1], M = IPERM[L - 1], N = IPERM[M - 1];
+  double XI[3];
+  XI[N - 1] = FVAL[NFACE - 1];
+  double FN[27], DFXI[81];
+  for (int lj = 0; lj < 3; ++lj) {
+    XI[M - 1] = GLL_X[lj];
+    const double WJ = GLL_W[lj];
+    for (int li = 0; li < 3; ++li) {
+      XI[L - 1] = GLL_X[li];
+      const double WI = GLL_W[li];
+      const double WSTAR = WI * WJ;
+      shape_3_27(XI[0], XI[1], XI[2], FN, DFXI);
+      double VR[3], VS[3];
+      for (int i = 0; i < 3; ++i) {
+        double sr = 0.0, ss = 0.0;
+        for (int n = 0; n < 27; ++n) { sr += XT[i + 3 * n] * DFXI[n + 27 * (L - 1)]; ss += XT[i + 3 * n] * DFXI[n + 27 * (M - 1)]; }
+        VR[i] = sr; VS[i] = ss;
+      }
+      double VN[3] = {VR[1] * VS[2] - VR[2] * VS[1], VR[2] * VS[0] - VR[0] * VS[2], VR[0] * VS[1] - VR[1] * VS[0]};
+      const double DETJ_SURF = std::sqrt(VN[0] * VN[0] + VN[1] * VN[1] + VN[2] * VN[2]);
+      const double FAC0 = WSTAR * DETJ_SURF;
+      for (int I = 0; I < 27; ++I) {
+        const double H1 = FN[I];
+        EFS[I] += (0.0 * H1) * FAC0;
+        EFS[I + 27] += (0.0 * H1) * FAC0;
+        EFS[I + 54] += (1.0 * H1) * FAC0;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- ELEMENT_g_3D
+void element_g(const double* XT, const double* EU, const double* EP, double* g_mu, double* g_lam) {
+  for (int n = 0; n < 27; ++n) { g_mu[n] = 0.0; g_lam[n] = 0.0; }
+  QPoint q;
+  for (int lz = 0; lz < 3; ++lz)
+    for (int ly = 0; ly < 3; ++ly)
+      for (int lx = 0; lx < 3; ++lx) {
+        qpoint(XT, lx, ly, lz, q, true);
+        const double* DX = q.DFX;
+        const double u1x = dot27(EU, DX), u2x = dot27(EU + 27, DX), u3x = dot27(EU + 54, DX);
+        const double u1y = dot27(EU, DX + 27), u2y = dot27(EU + 27, DX + 27), u3y = dot27(EU + 54, DX + 27);
+        const double u1z = dot27(EU, DX + 54), u2z = dot27(EU + 27, DX + 54), u3z = dot27(EU + 54, DX + 54);
+        const double p1x = dot27(EP, DX), p2x = dot27(EP + 27, DX), p3x = dot27(EP + 54, DX);
+        const double p1y = dot27(EP, DX + 27), p2y = dot27(EP + 27, DX + 27), p3y = dot27(EP + 54, DX + 27);
+        const double p1z = dot27(EP, DX + 54), p2z = dot27(EP + 27, DX + 54), p3z = dot27(EP + 54, DX + 54);
+        const double sL = (u1x + u2y + u3z) * (p1x + p2y + p3z);
+        const double sM = 2.0 * (u1x * p1x + u2y * p2y + u3z * p3z) + (p1y + p2x) * (u1y + u2x) +
+                          (p1z + p3x) * (u1z + u3x) + (p3y + p2z) * (u3y + u2z);
+        for (int n = 0; n < 27; ++n) {
+          g_mu[n] = g_mu[n] + q.FN[n] * sM * q.fac0;
+          g_lam[n] = g_lam[n] + q.FN[n] * sL * q.fac0;
+        }
+      }
+}
+
+// ------------------------------------------------------------ TIME_FUNCTION
+double time_function(double T, int lcase, double fr) {
+  const double pi = 3.14159265358979323846264338327950288419716939937510;
+  switch (lcase) {
+    case 1: return std::sin(2.0 * pi * fr * T);
+    case 2: {
+      const double wr = 2.0 * pi * fr;
+      const double t_max = 6.0 * std::sqrt(6.0) / wr;
+      const double u = wr * T - 3.0 * std::sqrt(6.0);
+      if (T < t_max) {
+        const double FAC1 = (0.25 * (u * u) - 0.5) * std::exp(-0.25 * (u * u)) - 13.0 * std::exp(-13.5);
+        const double FAC2 = 0.50 + 13.0 * std::exp(-13.5);
+        return FAC1 / FAC2;
+      }
+      return 0.0;
+    }
+    case 3: {
+      const double G_MEAN = 0.06, G_SDEV = 0.00040;   // L_Gaussian = 40
+      return std::exp(-(T - G_MEAN) * (T - G_MEAN) / G_SDEV);
+    }
+    case 8: return 1.0;
+    case 9: return 0.0;
+  }
+  return 0.0;
+}
+
+// ========================================================== global system
+struct CSR {
+  i64 n = 0;
+  std::vector<i64> rowptr;
+  std::vector<int> col;
+  std::vector<double> val;
+};
+
+struct Trip { int r, c; double v; };
+
+// Build a CSR matrix from triplets listed in insertion order: duplicates are
+// summed in that order (PETSc ADD_VALUES), zero values were never inserted.
+void build_csr(i64 n, std::vector<std::vector<Trip>>& chunks, CSR& A) {
+  A.n = n;
+  std::vector<i64> cnt(n + 1, 0);
+  for (auto& ch : chunks)
+    for (auto& t : ch) cnt[t.r + 1]++;
+  for (i64 i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+  i64 tot = cnt[n];
+  std::vector<int> cc(tot);
+  std::vector<double> vv(tot);
+  std::vector<i64> pos(cnt.begin(), cnt.end() - 1);
+  for (auto& ch : chunks) {
+    for (auto& t : ch) { i64 p = pos[t.r]++; cc[p] = t.c; vv[p] = t.v; }
+    std::vector<Trip>().swap(ch);
+  }
+  // per row: stable sort by column, sum duplicates in insertion order
+  std::vector<i64> newcnt(n + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (i64 i = 0; i < n; ++i) {
+    i64 b = cnt[i], e = cnt[i + 1];
+    std::vector<std::pair<int, i64>> idx;
+    idx.reserve(e - b);
+    for (i64 p = b; p < e; ++p) idx.push_back({cc[p], p});
+    std::stable_sort(idx.begin(), idx.end(), [](const std::pair<int, i64>& a, const std::pair<int, i64>& c) { return a.first < c.first; });
+    std::vector<int> c2; std::vector<double> v2;
+    for (size_t k = 0; k < idx.size(); ++k) {
+      if (!c2.empty() && c2.back() == idx[k].first) v2.back() += vv[idx[k].second];
+      else { c2.push_back(idx[k].first); v2.push_back(vv[idx[k].second]); }
+    }
+    for (size_t k = 0; k < c2.size(); ++k) { cc[b + k] = c2[k]; vv[b + k] = v2[k]; }
+    newcnt[i + 1] = (i64)c2.size();
+  }
+  for (i64 i = 0; i < n; ++i) newcnt[i + 1] += newcnt[i];
+  A.rowptr = newcnt;
+  A.col.resize(newcnt[n]);
+  A.val.resize(newcnt[n]);
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i) {
+    i64 b = cnt[i], nb = newcnt[i], len = newcnt[i + 1] - newcnt[i];
+    for (i64 k = 0; k < len; ++k) { A.col[nb + k] = cc[b + k]; A.val[nb + k] = vv[b + k]; }
+  }
+}
+
+void transpose_csr(const CSR& A, CSR& T) {
+  T.n = A.n;
+  T.rowptr.assign(A.n + 1, 0);
+  for (size_t k = 0; k < A.col.size(); ++k) T.rowptr[A.col[k] + 1]++;
+  for (i64 i = 0; i < A.n; ++i) T.rowptr[i + 1] += T.rowptr[i];
+  T.col.resize(A.col.size());
+  T.val.resize(A.val.size());
+  std::vector<i64> pos(T.rowptr.begin(), T.rowptr.end() - 1);
+  for (i64 i = 0; i < A.n; ++i)            // rows ascending: keeps scatter order
+    for (i64 k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) {
+      i64 p = pos[A.col[k]]++;
+      T.col[p] = (int)i;
+      T.val[p] = A.val[k];
+    }
+}
+
+// z = y + A x  (MatMultAdd, SeqAIJ)
+void matmult_add(const CSR& A, const double* vals, const double* x, const double* y, double* z) {
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < A.n; ++i) {
+    double sum = y[i];
+    for (i64 k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) sum += vals[k] * x[A.col[k]];
+    z[i] = sum;
+  }
+}
+
+// z = y + A^T x (MatMultTransposeAdd, SeqAIJ scatter order) using the
+// explicit transpose T whose rows list the original rows in ascending order.
+void matmult_transpose_add(const CSR& T, const double* tvals, const double* x, const double* y, double* z) {
+#pragma omp parallel for schedule(static)
+  for (i64 c = 0; c < T.n; ++c) {
+    double sum = y[c];
+    for (i64 k = T.rowptr[c]; k < T.rowptr[c + 1]; ++k) sum += x[T.col[k]] * tvals[k];
+    z[c] = sum;
+  }
+}
+
+struct System {
+  i64 nnode = 0, nel = 0, neq = 0;
+  std::vector<double> xyz;         // nnode x 3 (row-major)
+  std::vector<i64> inod;           // nel x 27
+  std::vector<int> mtel;
+  std::vector<i64> id;             // nnode x 9, -1 inactive
+  std::vector<double> lam, mu, rho_el;
+  double pml[24];
+  CSR A[3];                        // 0 = K, 1 = C, 2 = G (raw assembled)
+  CSR AT[3];                       // explicit transposes
+  std::vector<i64> tmap[3];        // AT entry -> A entry
+  std::vector<double> diag_m;
+  bool have_t = false;
+};
+
+void element_xt(const System& S, i64 e, double* XT, double* lam27, double* mu27) {
+  for (int n = 0; n < 27; ++n) {
+    i64 k = S.inod[e * 27 + n];
+    for (int d = 0; d < 3; ++d) XT[d + 3 * n] = S.xyz[k * 3 + d];
+    if (lam27) lam27[n] = S.lam[k];
+    if (mu27) mu27[n] = S.mu[k];
+  }
+}
+
+inline void add_trip(std::vector<Trip>& v, i64 r, i64 c, double val) {
+  if (r < 0 || c < 0 || val == 0.0) return;
+  v.push_back({(int)r, (int)c, val});
+}
+
+void assemble(System& S) {
+  const i64 nel = S.nel;
+  std::vector<std::vector<Trip>> tk(nel), tc(nel), tg(nel);
+  std::vector<std::vector<std::pair<i64, double>>> dm_rd(nel), dm_pml(nel);
+#pragma omp parallel
+  {
+    std::vector<double> EK(81 * 81), EM(81 * 81), EN(162 * 162), EA(81 * 162), EAl(81 * 162);
+#pragma omp for schedule(dynamic, 16)
+    for (i64 e = 0; e < nel; ++e) {
+      double XT[81], l27[27], m27[27];
+      element_xt(S, e, XT, l27, m27);
+      i64 ND1[81], ND2[162];
+      for (int n = 0; n < 27; ++n) {
+        i64 k = S.inod[e * 27 + n];
+        for (int c = 0; c < 3; ++c) ND1[n + 27 * c] = S.id[k * 9 + c];
+        for (int c = 0; c < 6; ++c) ND2[n + 27 * c] = S.id[k * 9 + 3 + c];
+      }
+      const double RO = S.rho_el[e];
+      if (S.mtel[e] == 1) {
+        // ASSEM_REGULAR_DOMAIN_PETSC_3D: M (flag 201), K; row-oriented insertion of EK
+        element_m(XT, RO, S.pml, 201, EM.data());
+        element_k(XT, l27, m27, EK.data());
+        auto& v = tk[e];
+        for (int i = 0; i < 81; ++i)
+          for (int j = 0; j < 81; ++j) add_trip(v, ND1[i], ND1[j], EK[i + 81 * j]);
+        for (int i = 0; i < 81; ++i) if (ND1[i] >= 0) dm_rd[e].push_back({ND1[i], EM[i + 81 * i]});
+      } else {
+        // ASSEM_PML_PETSC_3D
+        const int mflag[3] = {203, 204, 205};   // C, K, G  <- EM_b, EM_c, EM_d
+        const int nflag[3] = {302, 303, 304};   //           EN_b, EN_c, EN_d
+        const int aflag[3] = {101, 102, 103};   //           EA_e, EA_p, EA_w
+        std::vector<Trip>* dst[3] = {&tc[e], &tk[e], &tg[e]};
+        for (int s = 0; s < 3; ++s) {
+          auto& v = *dst[s];
+          element_m(XT, RO, S.pml, mflag[s], EM.data());
+          for (int i = 0; i < 81; ++i)
+            for (int j = 0; j < 81; ++j) add_trip(v, ND1[i], ND1[j], EM[i + 81 * j]);
+          element_en(XT, S.pml, nflag[s], EN.data());
+          for (int i = 0; i < 162; ++i)
+            for (int j = 0; j < 162; ++j) add_trip(v, ND2[i], ND2[j], EN[i + 162 * j]);
+          element_ea(XT, l27, m27, S.pml, aflag[s], EA.data());
+          element_ea(XT, l27, m27, S.pml, aflag[s] + 3, EAl.data());
+          // MatSetValuesLocal(A, 162, ND2, 81, ND1, EA_upper): the column-major
+          // Fortran array read row-oriented gives A[ND2(i), ND1(j)] += EA(j, i)
+          for (int i = 0; i < 162; ++i)
+            for (int j = 0; j < 81; ++j) add_trip(v, ND2[i], ND1[j], EA[j + 81 * i]);
+          // MatSetValuesLocal(A, 81, ND1, 162, ND2, -transpose(EA_lower)):
+          // A[ND1(i), ND2(j)] += -EA_lower(i, j)
+          for (int i = 0; i < 81; ++i)
+            for (int j = 0; j < 162; ++j) add_trip(v, ND1[i], ND2[j], -EAl[i + 81 * j]);
+        }
+        element_m(XT, RO, S.pml, 202, EM.data());
+        element_en(XT, S.pml, 301, EN.data());
+        for (int i = 0; i < 81; ++i) if (ND1[i] >= 0) dm_pml[e].push_back({ND1[i], EM[i + 81 * i]});
+        for (int i = 0; i < 162; ++i) if (ND2[i] >= 0) dm_pml[e].push_back({ND2[i], EN[i + 162 * i]});
+      }
+    }
+  }
+  build_csr(S.neq, tk, S.A[0]);
+  build_csr(S.neq, tc, S.A[1]);
+  build_csr(S.neq, tg, S.A[2]);
+  // VecSetValuesLocal ADD in element order: all RD elements, then all PML
+  S.diag_m.assign(S.neq, 0.0);
+  for (i64 e = 0; e < nel; ++e) for (auto& p : dm_rd[e]) S.diag_m[p.first] += p.second;
+  for (i64 e = 0; e < nel; ++e) for (auto& p : dm_pml[e]) S.diag_m[p.first] += p.second;
+}
+
+void ensure_transposes(System& S) {
+  if (S.have_t) return;
+  for (int w = 0; w < 3; ++w) {
+    CSR& A = S.A[w];
+    // transpose with an index map so scaled value arrays can be permuted
+    CSR& T = S.AT[w];
+    T.n = A.n;
+    T.rowptr.assign(A.n + 1, 0);
+    for (size_t k = 0; k < A.col.size(); ++k) T.rowptr[A.col[k] + 1]++;
+    for (i64 i = 0; i < A.n; ++i) T.rowptr[i + 1] += T.rowptr[i];
+    T.col.resize(A.col.size());
+    T.val.resize(A.val.size());
+    S.tmap[w].resize(A.col.size());
+    std::vector<i64> pos(T.rowptr.begin(), T.rowptr.end() - 1);
+    for (i64 i = 0; i < A.n; ++i)
+      for (i64 k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) {
+        i64 p = pos[A.col[k]]++;
+        T.col[p] = (int)i;
+        T.val[p] = A.val[k];
+        S.tmap[w][p] = k;
+      }
+  }
+  S.have_t = true;
+}
+
+// ---- PETSc Vec helpers
+void vset(std::vector<double>& x, double a) { std::fill(x.begin(), x.end(), a); }
+void vcopy(const std::vector<double>& x, std::vector<double>& y) { y = x; }
+// VecWAXPY(w, a, x, y): w = y + a*x
+void waxpy(std::vector<double>& w, double a, const std::vector<double>& x, const std::vector<double>& y) {
+  const i64 n = (i64)w.size();
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i) w[i] = y[i] + a * x[i];
+}
+// VecAXPY(y, a, x): y = y + a*x
+void axpy(std::vector<double>& y, double a, const std::vector<double>& x) {
+  const i64 n = (i64)y.size();
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i) y[i] = y[i] + a * x[i];
+}
+// VecSet(h,0); VecMAXPY(h, 4, [1,2,2,1], [k1..k4])
+void maxpy4(std::vector<double>& h, const std::vector<double>& k1, const std::vector<double>& k2,
+            const std::vector<double>& k3, const std::vector<double>& k4) {
+  const i64 n = (i64)h.size();
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < n; ++i) {
+    const double s1 = 1.0 * k1[i], s2 = 2.0 * k2[i], s3 = 2.0 * k3[i], s4 = 1.0 * k4[i];
+    h[i] = 0.0 + (s1 + s2 + s3 + s4);
+  }
+}
+
+}  // namespace
+
+// ================================================================ C API
+extern "C" {
+
+void or_shape_3_27(double x1, double x2, double x3, double* fn, double* dfxi) { shape_3_27(x1, x2, x3, fn, dfxi); }
+void or_pml_finder_3d(const double* xg, const double* pml_param, double* pab) { pml_finder_3d(xg, pml_param, pab); }
+void or_element_k(const double* xt, const double* lam, const double* mu, double* ek) { element_k(xt, lam, mu, ek); }
+void or_element_m(const double* xt, double rho, const double* pp, int iflag, double* em) { element_m(xt, rho, pp, iflag, em); }
+void or_element_en(const double* xt, const double* pp, int iflag, double* en) { element_en(xt, pp, iflag, en); }
+void or_element_ea(const double* xt, const double* lam, const double* mu, const double* pp, int iflag, double* ea) {
+  element_ea(xt, lam, mu, pp, iflag, ea);
+}
+void or_element_fs(const double* xt, int nface, double* efs) { element_fs(xt, nface, efs); }
+void or_element_g(const double* xt, const double* eu, const double* ep, double* g_mu, double* g_lam) {
+  element_g(xt, eu, ep, g_mu, g_lam);
+}
+double or_time_function(double t, int lcase, double fr) { return time_function(t, lcase, fr); }
+
+void or_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+int or_get_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void* or_system_create(i64 nnode, i64 nel, const double* xyz, const i64* inod, const int* mtel, const i64* id,
+                       i64 neq, const double* lam, const double* mu, const double* rho_el, const double* pml_param) {
+  System* S = new System;
+  S->nnode = nnode; S->nel = nel; S->neq = neq;
+  S->xyz.assign(xyz, xyz + nnode * 3);
+  S->inod.assign(inod, inod + nel * 27);
+  S->mtel.assign(mtel, mtel + nel);
+  S->id.assign(id, id + nnode * 9);
+  S->lam.assign(lam, lam + nnode);
+  S->mu.assign(mu, mu + nnode);
+  S->rho_el.assign(rho_el, rho_el + nel);
+  std::memcpy(S->pml, pml_param, sizeof(double) * 24);
+  assemble(*S);
+  return S;
+}
+
+void or_system_destroy(void* s) { delete (System*)s; }
+
+i64 or_nnz(void* s, int which) { return (i64)((System*)s)->A[which].col.size(); }
+
+void or_get_csr(void* s, int which, i64* rowptr, int* col, double* val) {
+  const CSR& A = ((System*)s)->A[which];
+  std::memcpy(rowptr, A.rowptr.data(), sizeof(i64) * (A.n + 1));
+  std::memcpy(col, A.col.data(), sizeof(int) * A.col.size());
+  std::memcpy(val, A.val.data(), sizeof(double) * A.val.size());
+}
+
+void or_get_diag_m(void* s, double* m) {
+  System* S = (System*)s;
+  std::memcpy(m, S->diag_m.data(), sizeof(double) * S->neq);
+}
+
+// raw z = y + A x or z = y + A^T x
+void or_matmult_add(void* s, int which, int transpose, const double* x, const double* y, double* z) {
+  System* S = (System*)s;
+  if (!transpose) {
+    matmult_add(S->A[which], S->A[which].val.data(), x, y, z);
+  } else {
+    ensure_transposes(*S);
+    matmult_transpose_add(S->AT[which], S->AT[which].val.data(), x, y, z);
+  }
+}
+
+// EXPLICIT_RK4_PETSC_PML_3D.  B: force vector (neq).  u_store: nstore x (nstep+1),
+// column n = stored entries of x2 at the start of step n (U_Store_Mapping).
+// The reference loop also advances once more after storing column NSTEP; that
+// state is never read and is not computed here.
+int or_rk4_forward(void* s, const double* Bin, int nstep, double dt, int lcase, double fr, i64 nstore,
+                   const i64* store_eq, double* u_store) {
+  System* S = (System*)s;
+  const i64 n = S->neq;
+  std::vector<double> iM(n);
+  for (i64 i = 0; i < n; ++i) iM[i] = 1.0 / S->diag_m[i];
+  // A <- -(M^-1 A): MatDiagonalScale(A, iM, NULL); MatScale(A, -1)
+  std::vector<double> val[3];
+  for (int w = 0; w < 3; ++w) {
+    const CSR& A = S->A[w];
+    val[w].resize(A.val.size());
+    for (i64 i = 0; i < n; ++i)
+      for (i64 k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) val[w][k] = (A.val[k] * iM[i]) * -1.0;
+  }
+  std::vector<double> B(n);
+  for (i64 i = 0; i < n; ++i) B[i] = Bin[i] * iM[i];
+  std::vector<double> x1(n, 0.0), x2(n, 0.0), x3(n, 0.0), help(n);
+  std::vector<double> k1t(n), k1m(n), k1b(n), k2t(n), k2m(n), k2b(n), k3t(n), k3m(n), k3b(n), k4t(n), k4m(n), k4b(n);
+  const CSR &K = S->A[0], &C = S->A[1], &G = S->A[2];
+  for (int istep = 0; istep <= nstep; ++istep) {
+    const double T = (double)istep * dt;
+    for (i64 r = 0; r < nstore; ++r) u_store[r + nstore * (i64)istep] = x2[store_eq[r]];
+    if (istep == nstep) break;
+    const double fac7 = time_function(T, lcase, fr);
+    const double fac8 = time_function(T + dt, lcase, fr);
+    const double fac9 = time_function(T + 0.5 * dt, lcase, fr);
+    for (i64 i = 0; i < n; ++i) {
+      k1b[i] = 0.0 + fac7 * B[i];
+      k4b[i] = 0.0 + fac8 * B[i];
+      k2b[i] = 0.0 + fac9 * B[i];
+      k3b[i] = 0.0 + fac9 * B[i];
+    }
+    vcopy(x2, k1t);
+    vcopy(x3, k1m);
+    matmult_add(C, val[1].data(), x3.data(), k1b.data(), k1b.data());
+    matmult_add(K, val[0].data(), x2.data(), k1b.data(), k1b.data());
+    matmult_add(G, val[2].data(), x1.data(), k1b.data(), k1b.data());
+    waxpy(k2t, 0.5 * dt, k1m, x2);
+    waxpy(k2m, 0.5 * dt, k1b, x3);
+    waxpy(help, 0.5 * dt, k1t, x1);
+    matmult_add(C, val[1].data(), k2m.data(), k2b.data(), k2b.data());
+    matmult_add(K, val[0].data(), k2t.data(), k2b.data(), k2b.data());
+    matmult_add(G, val[2].data(), help.data(), k2b.data(), k2b.data());
+    waxpy(k3t, 0.5 * dt, k2m, x2);
+    waxpy(k3m, 0.5 * dt, k2b, x3);
+    waxpy(help, 0.5 * dt, k2t, x1);
+    matmult_add(C, val[1].data(), k3m.data(), k3b.data(), k3b.data());
+    matmult_add(K, val[0].data(), k3t.data(), k3b.data(), k3b.data());
+    matmult_add(G, val[2].data(), help.data(), k3b.data(), k3b.data());
+    waxpy(k4t, dt, k3m, x2);
+    waxpy(k4m, dt, k3b, x3);
+    waxpy(help, dt, k3t, x1);
+    matmult_add(C, val[1].data(), k4m.data(), k4b.data(), k4b.data());
+    matmult_add(K, val[0].data(), k4t.data(), k4b.data(), k4b.data());
+    matmult_add(G, val[2].data(), help.data(), k4b.data(), k4b.data());
+    maxpy4(help, k1t, k2t, k3t, k4t);
+    axpy(x1, dt / 6.0, help);
+    maxpy4(help, k1m, k2m, k3m, k4m);
+    axpy(x2, dt / 6.0, help);
+    maxpy4(help, k1b, k2b, k3b, k4b);
+    axpy(x3, dt / 6.0, help);
+  }
+  return 0;
+}
+
+// EXPLICIT_RK4_PETSC_PML_3D_Adjoint_OtD.  rec_eq: EqDis (neq numbering),
+// rec_row: row of each receiver equation in the store
+// (EqDis_MAP_U_Store_Numbers_Global), dis_meas: (nstep+1) x nrec, row n =
+// measured data at t_n.  p_store (nstore x (nstep+1)) receives x2 at the start
+// of each backward step ISTEP = NSTEP..1.
+int or_rk4_adjoint(void* s, int nstep, double dt, i64 nstore, const i64* store_eq, const double* u_store,
+                   int nrec, const i64* rec_eq, const i64* rec_row, const double* dis_meas, double* p_store) {
+  System* S = (System*)s;
+  ensure_transposes(*S);
+  const i64 n = S->neq;
+  std::vector<double> M(S->diag_m), iM(n);
+  for (i64 i = 0; i < n; ++i) iM[i] = 1.0 / M[i];
+  // forward scaling, then AC, AG <- -A; A <- diag(M) A diag(iM)
+  std::vector<double> tval[3];
+  for (int w = 0; w < 3; ++w) {
+    const CSR& A = S->A[w];
+    std::vector<double> v(A.val.size());
+    for (i64 i = 0; i < n; ++i)
+      for (i64 k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) {
+        double a = (A.val[k] * iM[i]) * -1.0;
+        if (w != 0) a = a * -1.0;
+        v[k] = (a * M[i]) * iM[A.col[k]];
+      }
+    tval[w].resize(v.size());
+    for (size_t p = 0; p < v.size(); ++p) tval[w][p] = v[S->tmap[w][p]];
+  }
+  const CSR &KT = S->AT[0], &CT = S->AT[1], &GT = S->AT[2];
+  std::vector<double> x1(n, 0.0), x2(n, 0.0), x3(n, 0.0), help(n), bmis(n);
+  std::vector<double> k1t(n), k1m(n, 0.0), k1b(n), k2t(n), k2m(n, 0.0), k2b(n), k3t(n), k3m(n, 0.0), k3b(n), k4t(n),
+      k4m(n, 0.0), k4b(n);
+  for (int istep = nstep; istep >= 1; --istep) {
+    for (i64 r = 0; r < nstore; ++r) p_store[r + nstore * (i64)istep] = x2[store_eq[r]];
+    // misfit source at t_n and t_{n-1}
+    for (int pass = 0; pass < 2; ++pass) {
+      const int ts = pass == 0 ? istep : istep - 1;
+      vset(bmis, 0.0);
+      for (int q = 0; q < nrec; ++q) {
+        const double um = u_store[rec_row[q] + nstore * (i64)ts] - dis_meas[(i64)ts * nrec + q];
+        bmis[rec_eq[q]] += um;
+      }
+      for (i64 i = 0; i < n; ++i) bmis[i] = bmis[i] * iM[i];
+      std::vector<double>& kb = pass == 0 ? k1b : k4b;
+      for (i64 i = 0; i < n; ++i) kb[i] = 0.0 + 1.0 * bmis[i];
+    }
+    for (i64 i = 0; i < n; ++i) {
+      const double v = 0.5 * k1b[i] + 0.5 * k4b[i];
+      k2b[i] = 0.0 + v;
+      k3b[i] = 0.0 + v;
+    }
+    vcopy(x2, k1t);
+    vcopy(x3, k1m);
+    matmult_transpose_add(CT, tval[1].data(), x3.data(), k1b.data(), k1b.data());
+    matmult_transpose_add(KT, tval[0].data(), x2.data(), k1b.data(), k1b.data());
+    matmult_transpose_add(GT, tval[2].data(), x1.data(), k1b.data(), k1b.data());
+    waxpy(k2t, -0.5 * dt, k1m, x2);
+    waxpy(k2m, -0.5 * dt, k1b, x3);
+    waxpy(help, -0.5 * dt, k1t, x1);
+    matmult_transpose_add(CT, tval[1].data(), k2m.data(), k2b.data(), k2b.data());
+    matmult_transpose_add(KT, tval[0].data(), k2t.data(), k2b.data(), k2b.data());
+    matmult_transpose_add(GT, tval[2].data(), help.data(), k2b.data(), k2b.data());
+    waxpy(k3t, -0.5 * dt, k2m, x2);
+    waxpy(k3m, -0.5 * dt, k2b, x3);
+    waxpy(help, -0.5 * dt, k2t, x1);
+    matmult_transpose_add(CT, tval[1].data(), k3m.data(), k3b.data(), k3b.data());
+    matmult_transpose_add(KT, tval[0].data(), k3t.data(), k3b.data(), k3b.data());
+    matmult_transpose_add(GT, tval[2].data(), help.data(), k3b.data(), k3b.data());
+    waxpy(k4t, -dt, k3m, x2);
+    waxpy(k4m, -dt, k3b, x3);
+    waxpy(help, -dt, k3t, x1);
+    matmult_transpose_add(CT, tval[1].data(), k4m.data(), k4b.data(), k4b.data());
+    matmult_transpose_add(KT, tval[0].data(), k4t.data(), k4b.data(), k4b.data());
+    matmult_transpose_add(GT, tval[2].data(), help.data(), k4b.data(), k4b.data());
+    maxpy4(help, k1t, k2t, k3t, k4t);
+    axpy(x1, -(dt / 6.0), help);
+    maxpy4(help, k1m, k2m, k3m, k4m);
+    axpy(x2, -(dt / 6.0), help);
+    maxpy4(help, k1b, k2b, k3b, k4b);
+    axpy(x3, -(dt / 6.0), help);
+  }
+  return 0;
+}
+
+// EXPLICIT_RK4_PETSC_PML_3D_Control_OtD + Update_g + sign flip
+// (Program_Main.F90:412-413).  g_lambda, g_mu: nodal (nnode), zeroed here.
+void or_control(void* s, int nstep, double dt, i64 nstore, const i64* store_eq, const double* u_store,
+                const double* p_store, double* g_lambda, double* g_mu) {
+  System* S = (System*)s;
+  const i64 n = S->neq;
+  for (i64 k = 0; k < S->nnode; ++k) { g_lambda[k] = 0.0; g_mu[k] = 0.0; }
+  std::vector<double> U(n), P(n);
+  std::vector<i64> rd;
+  for (i64 e = 0; e < S->nel; ++e) if (S->mtel[e] == 1) rd.push_back(e);
+  const i64 nrd = (i64)rd.size();
+  std::vector<double> eg_mu(nrd * 27), eg_lam(nrd * 27);
+  for (int istep = 1; istep <= nstep - 1; ++istep) {
+    std::fill(U.begin(), U.end(), 0.0);
+    std::fill(P.begin(), P.end(), 0.0);
+    for (i64 r = 0; r < nstore; ++r) {
+      U[store_eq[r]] += u_store[r + nstore * (i64)istep];
+      P[store_eq[r]] += p_store[r + nstore * (i64)istep];
+    }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (i64 a = 0; a < nrd; ++a) {
+      const i64 e = rd[a];
+      double XT[81], EU[81], EP[81];
+      element_xt(*S, e, XT, nullptr, nullptr);
+      for (int nn = 0; nn < 27; ++nn) {
+        const i64 k = S->inod[e * 27 + nn];
+        for (int c = 0; c < 3; ++c) {
+          const i64 q = S->id[k * 9 + c];
+          EU[nn + 27 * c] = U[q];
+          EP[nn + 27 * c] = P[q];
+        }
+      }
+      element_g(XT, EU, EP, &eg_mu[a * 27], &eg_lam[a * 27]);
+    }
+    // VecSetValuesLocal ADD in element order, E_g * Scale_L
+    for (i64 a = 0; a < nrd; ++a) {
+      const i64 e = rd[a];
+      for (int nn = 0; nn < 27; ++nn) {
+        const i64 k = S->inod[e * 27 + nn];
+        g_mu[k] += eg_mu[a * 27 + nn] * dt;
+        g_lambda[k] += eg_lam[a * 27 + nn] * dt;
+      }
+    }
+  }
+  for (i64 k = 0; k < S->nnode; ++k) { g_lambda[k] = g_lambda[k] * -1.0; g_mu[k] = g_mu[k] * -1.0; }
+}
+
+// Compute_Misfit: J = sum_{n=NSTEP..1} 0.5 * |u_rec(t_n) - d_n|^2 * dt
+double or_misfit(int nstep, double dt, int nrec, const i64* rec_row, const double* u_store, i64 nstore,
+                 const double* dis_meas) {
+  double cost = 0.0;
+  for (int istep = nstep; istep >= 1; --istep) {
+    double c = 0.0;
+    for (int q = 0; q < nrec; ++q) {
+      const double um = u_store[rec_row[q] + nstore * (i64)istep] - dis_meas[(i64)istep * nrec + q];
+      c += um * um;
+    }
+    cost = cost + 0.5 * c * dt;
+  }
+  return cost;
+}
+
+}  // extern "C"
