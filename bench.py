@@ -1,0 +1,191 @@
+"""Benchmark: one forward + OtD adjoint + lambda/mu gradient solve per step.
+
+Workload: 64 x 64 x 32 elements of 1 m with 6-element PML (6.26 M DOFs incl.
+PML stresses), three layers Vs 150/250/400 m/s, Vp = 2 Vs, rho 1900/2000/2100,
+15 Hz Ricker at the centre surface node, 4000 RK4 steps of 1e-4 s, 400
+surface receivers.  Observed data come from the model with a -30 % Gaussian
+Vs anomaly; the gradient is evaluated at the model without it.  Synthetic
+data, FP64 throughout, all inputs resident in HBM before the timed region.
+
+Prints one JSON line.  value = DOF-steps/s = neq * (forward + adjoint steps)
+per second of the whole forward + adjoint + gradient solve.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--nstep", type=int, default=4000, help="RK4 steps per solve (workload: 4000)")
+    p.add_argument("--history", choices=["store", "checkpoint"], default="store")
+    p.add_argument("--segment", type=int, default=64)
+    p.add_argument("--tz", type=int, default=0)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-steps", type=int, default=12, help="oracle RK4 steps timed (each direction)")
+    p.add_argument("--copy-calib", action="store_true", help="also run a plain 1 GiB device copy (PMC scale)")
+    return p.parse_args()
+
+
+def stage_bytes(neq, nnode):
+    """Algorithmic bytes of one launch of the stage kernel for RK4 stage s:
+    stage s reads its 2+s input vectors (x1, x2, x3, g1..g_{s-1}), M^-1 and
+    lambda, mu; stages 1-3 write g_s; stage 4 writes the updated x1, x2, x3."""
+    b = {}
+    for s in (1, 2, 3, 4):
+        reads = (2 + s) * neq + neq + 2 * nnode
+        writes = neq if s < 4 else 3 * neq
+        b[s] = 8.0 * (reads + writes)
+    return b
+
+
+def cpu_baseline(cfg, nsteps):
+    """The oracle (CSR + PETSc-sequence RK4, OpenMP) on the same mesh and model,
+    timed over `nsteps` forward and `nsteps` adjoint steps; assembly excluded."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = len(os.sched_getaffinity(0))
+    O.lib().or_set_threads(threads)
+    mesh, model = cfg["mesh"], cfg["start"]
+    s = O.System(mesh, model)
+    B = np.zeros(mesh.neq)
+    B[cfg["src_eq"]] = 1.0
+    t0 = time.perf_counter()
+    U = s.forward(B, nsteps, cfg["dt"], lcase=2, fr=cfg["fr"])
+    t1 = time.perf_counter()
+    dmeas = np.zeros((nsteps + 1, len(cfg["rec_eq"])))
+    s.adjoint(U, nsteps, cfg["dt"], cfg["rec_eq"], dmeas)      # builds the transposes on first use
+    t2 = time.perf_counter()
+    s.adjoint(U, nsteps, cfg["dt"], cfg["rec_eq"], dmeas)
+    t3 = time.perf_counter()
+    s.close()
+    wall = (t1 - t0) + (t3 - t2)
+    return {"value": mesh.neq * 2 * nsteps / wall, "unit": "DOF-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle on the same 64x64x32 mesh and model, {nsteps} forward + {nsteps} adjoint "
+                      f"RK4 steps, assembly excluded ({wall:.1f} s)"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (shots would shard across GPUs "
+              f"with one all-reduce of the gradient per iteration)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from hirensi_amd import solver as S
+    from hirensi_amd.mesh import config2, receiver_dofs
+
+    dev = torch.device("cuda:0")
+    true_cfg = config2(with_anomaly=True)
+    start_cfg = config2(with_anomaly=False)
+    mesh = true_cfg["mesh"]
+    nstep, dt, fr = a.nstep, true_cfg["dt"], true_cfg["fr"]
+    rec_eq = receiver_dofs(mesh, true_cfg["rec_nodes"])
+    src_eq = np.array([mesh.dof(true_cfg["src_node"], 2)], dtype=np.int64)
+    src_val = np.array([1.0])
+    ftab = S.time_table(nstep, dt, 2, fr)
+    nrec = len(rec_eq)
+    history = S.HR_HISTORY_STORE if a.history == "store" else S.HR_HISTORY_CHECKPOINT
+
+    def T(x, dtype=torch.float64):
+        return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=dev)
+
+    ctx = S.Context(mesh, device_ptrs=True, tz=a.tz)
+    d_ftab, d_src, d_val, d_rec = T(ftab), T(src_eq, torch.int64), T(src_val), T(rec_eq, torch.int64)
+    d_obs = torch.zeros((nstep + 1, nrec), dtype=torch.float64, device=dev)
+    d_tr = torch.zeros_like(d_obs)
+    d_gl = torch.zeros(mesh.nnode, dtype=torch.float64, device=dev)
+    d_gm = torch.zeros_like(d_gl)
+    d_mis = torch.zeros(1, dtype=torch.float64, device=dev)
+    # observed data from the true model (untimed set-up)
+    tm = true_cfg["model"]
+    ctx.set_material_ptrs(T(tm.lam), T(tm.mu), T(tm.rho))
+    S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, d_ftab, d_src, d_val, d_rec, history=S.HR_HISTORY_NONE,
+                                traces=d_obs)
+    sm = start_cfg["model"]
+    ctx.set_material_ptrs(T(sm.lam), T(sm.mu), T(sm.rho))
+    torch.cuda.synchronize()
+
+    def gradient():
+        S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, d_ftab, d_src, d_val, d_rec, history=history,
+                                    segment=a.segment, traces=d_tr)
+        S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, d_rec, d_obs, d_gl, d_gm, misfit_out=d_mis)
+
+    for _ in range(a.warmup):
+        gradient()
+    torch.cuda.synchronize()
+    L = ctx.L
+    L.hr_set_profiling(ctx.h, 2)
+    L.hr_set_profiling(ctx.h, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        gradient()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    n_l = (np.zeros(5, dtype=np.int64))
+    ms_l = np.zeros(5)
+    L.hr_get_kernel_stats(ctx.h, n_l.ctypes.data, ms_l.ctypes.data)
+    L.hr_set_profiling(ctx.h, 0)
+    wall = (t1 - t0) / a.steps
+    misfit = float(d_mis.item())
+    gnorm = float(torch.linalg.norm(d_gm).item())
+    neq = ctx.neq
+    dof_steps = neq * 2.0 * nstep
+    value = dof_steps / wall
+    # roofline of the stage kernel: algorithmic bytes per launch / average duration
+    sb = stage_bytes(neq, mesh.nnode)
+    launches = int(n_l[1:].sum())
+    kern_ms = float(ms_l[1:].sum())
+    alg_bytes = sum(sb[s] * n_l[s] for s in (1, 2, 3, 4))
+    avg_us = kern_ms * 1e3 / launches
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    out = {
+        "metric": "fwd+adjoint gradient DOF-steps/s (64x64x32, 4000 steps, 1 GPU)",
+        "value": value,
+        "unit": "DOF-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded layered model; observed traces from the anomaly model)",
+        "config": {"workload": "fwd+adj+grad, 64x64x32 el, PML 6, 4000 RK4 steps, 400 receivers",
+                   "neq": neq, "nnode": mesh.nnode, "nstep": nstep, "history": a.history,
+                   "recompute_ratio": ctx.recompute_ratio(), "misfit": misfit, "g_mu_norm": gnorm,
+                   "gradient_wall_s": wall, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "stage_kernel<4,4>", "launches": launches, "avg_launch_us": avg_us,
+                     "alg_bytes_per_launch": alg_bytes / launches,
+                     "stage_kernel_share_of_wall": kern_ms * 1e-3 / (wall * a.steps)},
+    }
+    if a.copy_calib:
+        x = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+    ctx.close()
+    if not a.no_cpu_baseline:
+        cfg = {"mesh": mesh, "start": sm, "src_eq": src_eq, "rec_eq": rec_eq, "dt": dt, "fr": fr}
+        out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,972 @@
+// Host side of the C ABI declared in include/hirensi.h: context, mesh and
+// DOF model, node coefficients, stage-kernel launches and the RK4 drivers.
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstdarg>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "hr_device.h"
+#include "../../include/hirensi.h"
+
+#pragma clang fp contract(off)
+
+
+using namespace hr;
+
+namespace {
+constexpr int TX = 4, TY = 4;
+constexpr int DEFAULT_TZ = 8;
+
+struct HipError {
+  hipError_t e;
+};
+
+struct Shell {
+  double s = 1.0;
+  double* l = nullptr;   // device, neq (nullptr = identity)
+  double* r = nullptr;
+};
+
+}  // namespace
+
+struct hr_ctx {
+  std::string err;
+  int flags = 0;
+  Geo g{};
+  long long neq = 0;
+  std::vector<long long> eq_slot;          // host copy of the numbering
+  long long* d_eq_slot = nullptr;
+  // regular-domain node box
+  int box0[3] = {0, 0, 0}, boxn[3] = {0, 0, 0};
+  long long nbox = 0;
+  // node data (device)
+  double *d_xc = nullptr, *d_yc = nullptr, *d_zc = nullptr;
+  double *d_lam = nullptr, *d_mu = nullptr, *d_minv = nullptr, *d_rwp = nullptr, *d_wps = nullptr;
+  std::vector<double> h_minv_u, h_wps, h_a;  // host copies for M^-1 of sources, diag M
+  bool have_material = false, have_pml = false;
+  // state (device, 9*NN each)
+  double* x[3] = {nullptr, nullptr, nullptr};
+  double* y[3] = {nullptr, nullptr, nullptr};
+  double* gs[3] = {nullptr, nullptr, nullptr};
+  double* tmp = nullptr;                    // lattice scratch for hr_apply
+  double* tmp2 = nullptr;
+  Shell shell[3];
+  // forward record
+  int f_nstep = 0, f_nrec = 0, f_history = 0, f_segment = 0;
+  double f_dt = 0.0;
+  double* d_traces = nullptr;
+  size_t traces_cap = 0;
+  std::vector<long long> rec_slot_h;
+  long long* d_rec_slot = nullptr;
+  double* d_hist = nullptr;                 // store mode: (nstep+1) x 3 x nbox
+  size_t hist_cap = 0;
+  std::vector<double*> ckpt;                // checkpoint states (3 x 9NN each)
+  double* d_seg = nullptr;                  // checkpoint mode segment history
+  size_t seg_cap = 0;
+  // forward sources for recomputation
+  std::vector<int> src_tile_ptr_h;
+  int* d_src_tile_ptr = nullptr;
+  long long* d_src_node = nullptr;
+  int* d_src_comp = nullptr;
+  double* d_src_m = nullptr;
+  double* d_src_val = nullptr;
+  std::vector<double> ftab;
+  // adjoint sources
+  int* d_rec_tile_ptr = nullptr;
+  long long* d_rec_node = nullptr;
+  int* d_rec_comp = nullptr;
+  double* d_rec_m = nullptr;
+  long long* d_rec_perm = nullptr;          // tile-sorted entry -> receiver index
+  double *d_obs = nullptr, *d_gl = nullptr, *d_gm = nullptr, *d_part = nullptr, *d_scalar = nullptr;
+  double *d_tr_sorted = nullptr, *d_obs_sorted = nullptr;
+  size_t obs_cap = 0, sorted_cap = 0;
+  double recompute = 1.0;
+  hipStream_t stream = nullptr;
+  // stage-kernel timing
+  int profiling = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
+  long long prof_n[5] = {0, 0, 0, 0, 0};
+  double prof_ms[5] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(hr_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPC(call)                                                                                   \
+  do {                                                                                               \
+    hipError_t _e = (call);                                                                          \
+    if (_e != hipSuccess) return fail(ctx, (int)_e, "%s failed: %s", #call, hipGetErrorString(_e)); \
+  } while (0)
+
+template <class T>
+int dalloc(hr_ctx* ctx, T** p, size_t n) {
+  if (*p) { (void)hipFree(*p); *p = nullptr; }
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(ctx, (int)e, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  return 0;
+}
+
+// host <-> device copies honouring HR_DEVICE_PTRS
+int to_dev(hr_ctx* ctx, double* dst, const double* src, size_t n) {
+  HIPC(hipMemcpyAsync(dst, src, n * sizeof(double),
+                      (ctx->flags & HR_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+  return 0;
+}
+int from_dev(hr_ctx* ctx, double* dst, const double* src, size_t n) {
+  HIPC(hipMemcpyAsync(dst, src, n * sizeof(double),
+                      (ctx->flags & HR_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+  if (!(ctx->flags & HR_DEVICE_PTRS)) HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// ----------------------------------------------------------- DOF rules (host)
+bool h_u_active(const Geo& g, int i, int j, int k) {
+  return !(i == 0 || j == 0 || k == 0 || i == g.NX - 1 || j == g.NY - 1);
+}
+bool h_pml_node(const Geo& g, int i, int j, int k) {
+  return !(i > 2 * g.rlo[0] && i < 2 * g.rhi[0] && j > 2 * g.rlo[1] && j < 2 * g.rhi[1] && k > 2 * g.rlo[2]);
+}
+int h_s_mask(const Geo& g, int i, int j, int k) {
+  if (!h_pml_node(g, i, j, k)) return 0;
+  if (g.top_rule && k == g.NZ - 1 && !((i & 1) && (j & 1))) return 0x3f & ~((1 << 2) | (1 << 4) | (1 << 5));
+  return 0x3f;
+}
+bool h_active(const Geo& g, int i, int j, int k, int c) {
+  return c < 3 ? h_u_active(g, i, j, k) : ((h_s_mask(g, i, j, k) >> (c - 3)) & 1);
+}
+
+void h_stretch(const double* P, double x, double y, double z, double s[6]) {
+  s[0] = s[1] = s[2] = 1.0;
+  s[3] = s[4] = s[5] = 0.0;
+  if (x >= P[18]) { double f = (x - P[18]) * 1.0 / P[12]; s[0] = 1.0 + P[0] * (f * f); s[3] = 0.0 + P[6] * (f * f); }
+  else if (x <= P[19]) { double f = (x - P[19]) * -1.0 / P[13]; s[0] = 1.0 + P[1] * (f * f); s[3] = 0.0 + P[7] * (f * f); }
+  if (y >= P[20]) { double f = (y - P[20]) * 1.0 / P[14]; s[1] = 1.0 + P[2] * (f * f); s[4] = 0.0 + P[8] * (f * f); }
+  else if (y <= P[21]) { double f = (y - P[21]) * -1.0 / P[15]; s[1] = 1.0 + P[3] * (f * f); s[4] = 0.0 + P[9] * (f * f); }
+  if (z <= P[23]) { double f = (z - P[23]) * -1.0 / P[17]; s[2] = 1.0 + P[5] * (f * f); s[5] = 0.0 + P[11] * (f * f); }
+}
+
+long long lat(const Geo& g, int i, int j, int k) {
+  return (long long)i + (long long)g.NX * ((long long)j + (long long)g.NY * k);
+}
+
+int tile_of(const Geo& g, int i, int j, int k) {
+  const int tx = std::min(i / (2 * TX), g.tiles[0] - 1);
+  const int ty = std::min(j / (2 * TY), g.tiles[1] - 1);
+  const int tz = std::min(k / (2 * g.tz), g.tiles[2] - 1);
+  return tx + g.tiles[0] * (ty + g.tiles[1] * tz);
+}
+
+int ntiles(const Geo& g) { return g.tiles[0] * g.tiles[1] * g.tiles[2]; }
+
+// Sparse source grouped by owning tile.  entries: (lattice node, comp, M^-1)
+struct SrcBuild {
+  std::vector<int> ptr;
+  std::vector<long long> node;
+  std::vector<int> comp;
+  std::vector<double> m;
+  std::vector<long long> perm;   // sorted position -> input index
+};
+
+SrcBuild build_src(hr_ctx* ctx, int n, const long long* eq) {
+  const Geo& g = ctx->g;
+  std::vector<std::pair<int, int>> key(n);
+  for (int q = 0; q < n; ++q) {
+    const long long slot = ctx->eq_slot[eq[q]];
+    const int c = (int)(slot / g.NN);
+    const long long nd = slot - c * g.NN;
+    const int i = (int)(nd % g.NX), j = (int)((nd / g.NX) % g.NY), k = (int)(nd / ((long long)g.NX * g.NY));
+    key[q] = {tile_of(g, i, j, k), q};
+  }
+  std::stable_sort(key.begin(), key.end());
+  SrcBuild s;
+  const int nt = ntiles(g);
+  s.ptr.assign(nt + 1, 0);
+  for (auto& kq : key) s.ptr[kq.first + 1]++;
+  for (int t = 0; t < nt; ++t) s.ptr[t + 1] += s.ptr[t];
+  for (auto& kq : key) {
+    const long long slot = ctx->eq_slot[eq[kq.second]];
+    const int c = (int)(slot / g.NN);
+    const long long nd = slot - c * g.NN;
+    s.node.push_back(nd);
+    s.comp.push_back(c);
+    s.m.push_back(ctx->h_minv_u[nd]);
+    s.perm.push_back(kq.second);
+  }
+  return s;
+}
+
+template <class T>
+int upload(hr_ctx* ctx, T** d, const std::vector<T>& h) {
+  if (int rc = dalloc(ctx, d, h.size())) return rc;
+  if (!h.empty()) HIPC(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+hipEvent_t take_event(hr_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// fold finished event pairs into the per-stage totals
+int drain_events(hr_ctx* ctx) {
+  if (ctx->ev_open.empty()) return 0;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  for (auto& p : ctx->ev_open) {
+    float ms = 0.0f;
+    HIPC(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    ctx->prof_n[p.first] += 1;
+    ctx->prof_ms[p.first] += ms;
+    ctx->ev_pool.push_back(p.second.first);
+    ctx->ev_pool.push_back(p.second.second);
+  }
+  ctx->ev_open.clear();
+  return 0;
+}
+
+int launch_stage(hr_ctx* ctx, StageArgs& A) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ctx->profiling) {
+    if (ctx->ev_open.size() >= 4096) if (int rc = drain_events(ctx)) return rc;
+    e0 = take_event(ctx);
+    e1 = take_event(ctx);
+    if (!e0 || !e1) return fail(ctx, 3, "hipEventCreate failed");
+    HIPC(hipEventRecord(e0, ctx->stream));
+  }
+  HIPC(launch_stage(A, ntiles(ctx->g), ctx->stream));
+  if (ctx->profiling) {
+    HIPC(hipEventRecord(e1, ctx->stream));
+    ctx->ev_open.push_back({A.stage, {e0, e1}});
+  }
+  return 0;
+}
+
+StageArgs base_args(hr_ctx* ctx) {
+  StageArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.g = ctx->g;
+  A.nd.xc = ctx->d_xc; A.nd.yc = ctx->d_yc; A.nd.zc = ctx->d_zc;
+  A.nd.lam = ctx->d_lam; A.nd.mu = ctx->d_mu;
+  A.nd.minv_u = ctx->d_minv; A.nd.rwp = ctx->d_rwp; A.nd.wps = ctx->d_wps;
+  A.sgn[0] = A.sgn[1] = A.sgn[2] = 1.0;
+  return A;
+}
+
+int check_ready(hr_ctx* ctx) {
+  if (!ctx) return 1;
+  if (!ctx->have_pml) return fail(ctx, 2, "hr_set_pml has not been called");
+  if (!ctx->have_material) return fail(ctx, 2, "hr_set_material has not been called");
+  return 0;
+}
+
+int zero_state(hr_ctx* ctx) {
+  const size_t n = (size_t)NCOMP * ctx->g.NN;
+  for (int a = 0; a < 3; ++a) {
+    HIPC(hipMemsetAsync(ctx->x[a], 0, n * sizeof(double), ctx->stream));
+    HIPC(hipMemsetAsync(ctx->y[a], 0, n * sizeof(double), ctx->stream));
+    HIPC(hipMemsetAsync(ctx->gs[a], 0, n * sizeof(double), ctx->stream));
+  }
+  return 0;
+}
+
+// One RK4 step on (x) -> (y), then swap.  dir = +1 forward, -1 adjoint.
+int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3], const Source* src4) {
+  const double hh = dir * (0.5 * dt), h = dir * dt, h6 = dir * (dt / 6.0);
+  for (int s = 1; s <= 4; ++s) {
+    StageArgs A = base_args(ctx);
+    A.stage = s;
+    A.set_mask = 7;
+    A.transpose = transpose;
+    for (int a = 0; a < 3; ++a) A.sgn[a] = sgn[a];
+    A.hh = hh; A.h = h; A.h6 = h6;
+    A.in[0] = ctx->x[0]; A.in[1] = ctx->x[1]; A.in[2] = ctx->x[2];
+    A.in[3] = ctx->gs[0]; A.in[4] = ctx->gs[1]; A.in[5] = ctx->gs[2];
+    if (s < 4) A.out = ctx->gs[s - 1];
+    else { A.xnew[0] = ctx->y[0]; A.xnew[1] = ctx->y[1]; A.xnew[2] = ctx->y[2]; }
+    if (src4 && src4[s - 1].tile_ptr) { A.src = src4[s - 1]; A.nsrc_tiles = 1; }
+    if (int rc = launch_stage(ctx, A)) return rc;
+  }
+  std::swap(ctx->x, ctx->y);
+  return 0;
+}
+
+int record_step(hr_ctx* ctx, int n) {
+  if (ctx->f_nrec > 0) {
+    HIPC(launch_record(ctx->f_nrec, ctx->d_rec_slot, ctx->x[1], ctx->d_traces + (size_t)n * ctx->f_nrec, ctx->stream));
+  }
+  return 0;
+}
+
+int store_slice(hr_ctx* ctx, double* dst) {
+  HIPC(launch_store_box(ctx->g, ctx->box0, ctx->boxn, ctx->x[1], dst, ctx->stream));
+  return 0;
+}
+
+// forward sources of step n: a = B values, ca = f
+void forward_sources(hr_ctx* ctx, int n, Source s4[4]) {
+  for (int s = 0; s < 4; ++s) {
+    std::memset(&s4[s], 0, sizeof(Source));
+    if (!ctx->d_src_tile_ptr) continue;
+    s4[s].tile_ptr = ctx->d_src_tile_ptr;
+    s4[s].node = ctx->d_src_node;
+    s4[s].comp = ctx->d_src_comp;
+    s4[s].m = ctx->d_src_m;
+    s4[s].a = ctx->d_src_val;
+    const double* f = &ctx->ftab[3 * (size_t)n];
+    s4[s].ca = (s == 0) ? f[0] : (s == 3 ? f[2] : f[1]);
+  }
+}
+
+int run_forward_steps(hr_ctx* ctx, int n0, int n1, double* seg_hist) {
+  // steps n0 .. n1-1 from the current state; stores slices n0..n1 into seg_hist when given
+  const double sgn[3] = {1.0, 1.0, 1.0};
+  for (int n = n0; n < n1; ++n) {
+    if (seg_hist) if (int rc = store_slice(ctx, seg_hist + (size_t)(n - n0) * 3 * ctx->nbox)) return rc;
+    Source s4[4];
+    forward_sources(ctx, n, s4);
+    if (int rc = rk4_step(ctx, ctx->f_dt, +1, 0, sgn, s4)) return rc;
+  }
+  if (seg_hist) if (int rc = store_slice(ctx, seg_hist + (size_t)(n1 - n0) * 3 * ctx->nbox)) return rc;
+  return 0;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* hr_last_error(const hr_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+double hr_time_function(double T, int lcase, double fr) {
+  const double pi = 3.14159265358979323846264338327950288419716939937510;
+  switch (lcase) {
+    case 1: return std::sin(2.0 * pi * fr * T);
+    case 2: {
+      const double wr = 2.0 * pi * fr;
+      const double t_max = 6.0 * std::sqrt(6.0) / wr;
+      const double u = wr * T - 3.0 * std::sqrt(6.0);
+      if (T < t_max) {
+        const double f1 = (0.25 * (u * u) - 0.5) * std::exp(-0.25 * (u * u)) - 13.0 * std::exp(-13.5);
+        const double f2 = 0.50 + 13.0 * std::exp(-13.5);
+        return f1 / f2;
+      }
+      return 0.0;
+    }
+    case 3: return std::exp(-(T - 0.06) * (T - 0.06) / 0.00040);
+    case 8: return 1.0;
+    default: return 0.0;
+  }
+}
+
+int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
+  if (!out) return 1;
+  *out = nullptr;
+  hr_ctx* ctx = new hr_ctx;
+  *out = ctx;
+  if (!d) return fail(ctx, 1, "null mesh descriptor");
+  if (d->nex < 2 || d->ney < 2 || d->nez < 2) return fail(ctx, 1, "mesh must have at least 2 elements per axis");
+  for (int a = 0; a < 5; ++a)
+    if (d->npml[a] < 0) return fail(ctx, 1, "negative PML thickness");
+  if (d->npml[0] + d->npml[1] >= d->nex || d->npml[2] + d->npml[3] >= d->ney || d->npml[4] >= d->nez)
+    return fail(ctx, 1, "PML layers leave no regular domain");
+  if (!(d->h[0] > 0 && d->h[1] > 0 && d->h[2] > 0)) return fail(ctx, 1, "element edges must be positive");
+  ctx->flags = d->flags;
+  Geo& g = ctx->g;
+  g.nex = d->nex; g.ney = d->ney; g.nez = d->nez;
+  g.NX = 2 * g.nex + 1; g.NY = 2 * g.ney + 1; g.NZ = 2 * g.nez + 1;
+  g.NN = (long long)g.NX * g.NY * g.NZ;
+  g.rlo[0] = d->npml[0]; g.rhi[0] = g.nex - d->npml[1];
+  g.rlo[1] = d->npml[2]; g.rhi[1] = g.ney - d->npml[3];
+  g.rlo[2] = d->npml[4]; g.rhi[2] = g.nez;
+  g.top_rule = d->top_stress_bc ? 1 : 0;
+  g.tz = DEFAULT_TZ;
+  g.tiles[0] = (g.nex + TX - 1) / TX;
+  g.tiles[1] = (g.ney + TY - 1) / TY;
+  g.tiles[2] = (g.nez + g.tz - 1) / g.tz;
+  // affine element: DJ = diag(h/2); DETJ and DJI as in ELEMENT_K_3D
+  const double j11 = d->h[0] * 0.5, j22 = d->h[1] * 0.5, j33 = d->h[2] * 0.5;
+  const double detj = j11 * j22 * j33 + 0.0 * 0.0 * 0.0 + 0.0 * 0.0 * 0.0 - 0.0 * j22 * 0.0 - 0.0 * 0.0 * j11 - j33 * 0.0 * 0.0;
+  g.dxi[0] = (+j33 * j22 - 0.0 * 0.0) / detj;
+  g.dxi[1] = (+j33 * j11 - 0.0 * 0.0) / detj;
+  g.dxi[2] = (+j22 * j11 - 0.0 * 0.0) / detj;
+  const double w[3] = {0.33333333333333333333, 1.33333333333333333333, 0.33333333333333333333};
+  for (int c = 0; c < 3; ++c)
+    for (int b = 0; b < 3; ++b)
+      for (int a = 0; a < 3; ++a) g.wq[a + 3 * b + 9 * c] = (w[a] * w[b] * w[c]) * detj;
+  // regular-domain node box
+  for (int a = 0; a < 3; ++a) {
+    ctx->box0[a] = 2 * g.rlo[a];
+    ctx->boxn[a] = 2 * (g.rhi[a] - g.rlo[a]) + 1;
+  }
+  ctx->nbox = (long long)ctx->boxn[0] * ctx->boxn[1] * ctx->boxn[2];
+  // equation numbering: lattice order, active components interleaved
+  ctx->eq_slot.clear();
+  for (int k = 0; k < g.NZ; ++k)
+    for (int j = 0; j < g.NY; ++j)
+      for (int i = 0; i < g.NX; ++i)
+        for (int c = 0; c < NCOMP; ++c)
+          if (h_active(g, i, j, k, c)) ctx->eq_slot.push_back((long long)c * g.NN + lat(g, i, j, k));
+  ctx->neq = (long long)ctx->eq_slot.size();
+  HIPC(hipStreamCreate(&ctx->stream));
+  if (int rc = upload(ctx, &ctx->d_eq_slot, ctx->eq_slot)) return rc;
+  // coordinates (origin: centred in x, y; top surface at z = 0)
+  std::vector<double> xc(g.NX), yc(g.NY), zc(g.NZ);
+  const double xmin = -0.5 * g.nex * d->h[0], ymin = -0.5 * g.ney * d->h[1], zmin = -(double)g.nez * d->h[2];
+  for (int i = 0; i < g.NX; ++i) xc[i] = xmin + (double)i * (0.5 * d->h[0]);
+  for (int j = 0; j < g.NY; ++j) yc[j] = ymin + (double)j * (0.5 * d->h[1]);
+  for (int k = 0; k < g.NZ; ++k) zc[k] = zmin + (double)k * (0.5 * d->h[2]);
+  if (int rc = upload(ctx, &ctx->d_xc, xc)) return rc;
+  if (int rc = upload(ctx, &ctx->d_yc, yc)) return rc;
+  if (int rc = upload(ctx, &ctx->d_zc, zc)) return rc;
+  const size_t nf = (size_t)NCOMP * g.NN;
+  for (int a = 0; a < 3; ++a) {
+    if (int rc = dalloc(ctx, &ctx->x[a], nf)) return rc;
+    if (int rc = dalloc(ctx, &ctx->y[a], nf)) return rc;
+    if (int rc = dalloc(ctx, &ctx->gs[a], nf)) return rc;
+  }
+  if (int rc = dalloc(ctx, &ctx->tmp, nf)) return rc;
+  if (int rc = dalloc(ctx, &ctx->tmp2, nf)) return rc;
+  HIPC(hipMemset(ctx->tmp, 0, nf * sizeof(double)));
+  HIPC(hipMemset(ctx->tmp2, 0, nf * sizeof(double)));
+  if (int rc = zero_state(ctx)) return rc;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int hr_ctx_destroy(hr_ctx* ctx) {
+  if (!ctx) return 0;
+  auto F = [](void* p) { if (p) (void)hipFree(p); };
+  F(ctx->d_eq_slot); F(ctx->d_xc); F(ctx->d_yc); F(ctx->d_zc); F(ctx->d_lam); F(ctx->d_mu); F(ctx->d_minv);
+  F(ctx->d_rwp); F(ctx->d_wps);
+  for (int a = 0; a < 3; ++a) { F(ctx->x[a]); F(ctx->y[a]); F(ctx->gs[a]); F(ctx->shell[a].l); F(ctx->shell[a].r); }
+  F(ctx->tmp); F(ctx->tmp2); F(ctx->d_traces); F(ctx->d_rec_slot); F(ctx->d_hist); F(ctx->d_seg);
+  for (double* p : ctx->ckpt) F(p);
+  F(ctx->d_src_tile_ptr); F(ctx->d_src_node); F(ctx->d_src_comp); F(ctx->d_src_m); F(ctx->d_src_val);
+  F(ctx->d_rec_tile_ptr); F(ctx->d_rec_node); F(ctx->d_rec_comp); F(ctx->d_rec_m); F(ctx->d_rec_perm);
+  F(ctx->d_obs); F(ctx->d_gl); F(ctx->d_gm); F(ctx->d_part); F(ctx->d_scalar); F(ctx->d_tr_sorted);
+  F(ctx->d_obs_sorted);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->ev_open) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+int hr_get_sizes(hr_ctx* ctx, long long* neq, long long* nnode, long long* nstore) {
+  if (!ctx) return 1;
+  if (neq) *neq = ctx->neq;
+  if (nnode) *nnode = ctx->g.NN;
+  if (nstore) *nstore = 3 * ctx->nbox;
+  return 0;
+}
+
+int hr_get_store_eq(hr_ctx* ctx, long long* store_eq) {
+  if (!ctx || !store_eq) return fail(ctx, 1, "null argument");
+  // inverse numbering for the box
+  const Geo& g = ctx->g;
+  std::vector<long long> inv((size_t)NCOMP * g.NN, -1);
+  for (long long e = 0; e < ctx->neq; ++e) inv[ctx->eq_slot[e]] = e;
+  long long t = 0;
+  for (int c = 0; c < 3; ++c)
+    for (int bk = 0; bk < ctx->boxn[2]; ++bk)
+      for (int bj = 0; bj < ctx->boxn[1]; ++bj)
+        for (int bi = 0; bi < ctx->boxn[0]; ++bi)
+          store_eq[t++] = inv[(size_t)c * g.NN + lat(g, ctx->box0[0] + bi, ctx->box0[1] + bj, ctx->box0[2] + bk)];
+  return 0;
+}
+
+int hr_set_pml(hr_ctx* ctx, const double* p) {
+  if (!ctx || !p) return fail(ctx, 1, "null argument");
+  for (int f = 0; f < 6; ++f)
+    if (f != 4 && !(p[12 + f] > 0.0)) return fail(ctx, 1, "PML length of face %d must be positive", f + 1);
+  // the reference stops in a top PML zone (PML.F90:629-633): no node may lie above z0 of face 5
+  if (!(p[22] >= 0.0)) return fail(ctx, 1, "top PML zone (face 5) would contain nodes");
+  std::memcpy(ctx->g.pml, p, sizeof(double) * 24);
+  ctx->have_pml = true;
+  return 0;
+}
+
+int hr_set_material(hr_ctx* ctx, const double* lam, const double* mu, const double* rho_el) {
+  if (!ctx || !lam || !mu || !rho_el) return fail(ctx, 1, "null argument");
+  if (!ctx->have_pml) return fail(ctx, 2, "call hr_set_pml before hr_set_material");
+  const Geo& g = ctx->g;
+  const size_t NN = (size_t)g.NN, nel = (size_t)g.nex * g.ney * g.nez;
+  std::vector<double> hl(NN), hm(NN), hr(nel);
+  if (ctx->flags & HR_DEVICE_PTRS) {
+    HIPC(hipMemcpy(hl.data(), lam, NN * sizeof(double), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hm.data(), mu, NN * sizeof(double), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hr.data(), rho_el, nel * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::memcpy(hl.data(), lam, NN * sizeof(double));
+    std::memcpy(hm.data(), mu, NN * sizeof(double));
+    std::memcpy(hr.data(), rho_el, nel * sizeof(double));
+  }
+  for (size_t e = 0; e < nel; ++e)
+    if (!(hr[e] > 0.0)) return fail(ctx, 1, "element %zu has non-positive density", e);
+  std::vector<double> mrd(NN, 0.0), rwp(NN, 0.0), wps(NN, 0.0);
+  for (int ez = 0; ez < g.nez; ++ez)
+    for (int ey = 0; ey < g.ney; ++ey)
+      for (int ex = 0; ex < g.nex; ++ex) {
+        const size_t e = (size_t)ex + (size_t)g.nex * ((size_t)ey + (size_t)g.ney * ez);
+        const bool rd = ex >= g.rlo[0] && ex < g.rhi[0] && ey >= g.rlo[1] && ey < g.rhi[1] && ez >= g.rlo[2];
+        for (int c = 0; c < 3; ++c)
+          for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+              const long long n = lat(g, 2 * ex + a, 2 * ey + b, 2 * ez + c);
+              const double W = g.wq[a + 3 * b + 9 * c];
+              if (rd) mrd[n] += hr[e] * W;
+              else { rwp[n] += hr[e] * W; wps[n] += W; }
+            }
+      }
+  // coordinates
+  std::vector<double> xc(g.NX), yc(g.NY), zc(g.NZ);
+  HIPC(hipMemcpy(xc.data(), ctx->d_xc, g.NX * sizeof(double), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(yc.data(), ctx->d_yc, g.NY * sizeof(double), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(zc.data(), ctx->d_zc, g.NZ * sizeof(double), hipMemcpyDeviceToHost));
+  ctx->h_minv_u.assign(NN, 0.0);
+  ctx->h_a.assign(NN, 1.0);
+  for (int k = 0; k < g.NZ; ++k)
+    for (int j = 0; j < g.NY; ++j)
+      for (int i = 0; i < g.NX; ++i) {
+        const long long n = lat(g, i, j, k);
+        double s[6];
+        h_stretch(g.pml, xc[i], yc[j], zc[k], s);
+        const double a = s[0] * s[1] * s[2];
+        ctx->h_a[n] = a;
+        const double M = mrd[n] + a * rwp[n];
+        ctx->h_minv_u[n] = M > 0.0 ? 1.0 / M : 0.0;
+      }
+  ctx->h_wps = wps;
+  if (int rc = upload(ctx, &ctx->d_lam, hl)) return rc;
+  if (int rc = upload(ctx, &ctx->d_mu, hm)) return rc;
+  if (int rc = upload(ctx, &ctx->d_minv, ctx->h_minv_u)) return rc;
+  if (int rc = upload(ctx, &ctx->d_rwp, rwp)) return rc;
+  if (int rc = upload(ctx, &ctx->d_wps, wps)) return rc;
+  ctx->have_material = true;
+  return 0;
+}
+
+int hr_get_diag_m(hr_ctx* ctx, double* m) {
+  if (int rc = check_ready(ctx)) return rc;
+  const Geo& g = ctx->g;
+  std::vector<double> hm(ctx->neq);
+  for (long long e = 0; e < ctx->neq; ++e) {
+    const long long slot = ctx->eq_slot[e];
+    const int c = (int)(slot / g.NN);
+    const long long n = slot - c * g.NN;
+    if (c < 3) hm[e] = 1.0 / ctx->h_minv_u[n];
+    else hm[e] = (c < 6 ? 1.0 : 2.0) * ctx->h_a[n] * ctx->h_wps[n];
+  }
+  if (ctx->flags & HR_DEVICE_PTRS) HIPC(hipMemcpy(m, hm.data(), hm.size() * sizeof(double), hipMemcpyHostToDevice));
+  else std::memcpy(m, hm.data(), hm.size() * sizeof(double));
+  return 0;
+}
+
+int hr_scale(hr_ctx* ctx, int which, double s) {
+  if (!ctx || which < 0 || which > 2) return fail(ctx, 1, "bad operator index");
+  ctx->shell[which].s *= s;
+  return 0;
+}
+
+int hr_reset_scaling(hr_ctx* ctx, int which) {
+  if (!ctx || which < 0 || which > 2) return fail(ctx, 1, "bad operator index");
+  Shell& S = ctx->shell[which];
+  S.s = 1.0;
+  if (S.l) { (void)hipFree(S.l); S.l = nullptr; }
+  if (S.r) { (void)hipFree(S.r); S.r = nullptr; }
+  return 0;
+}
+
+int hr_diagonal_scale(hr_ctx* ctx, int which, const double* l, const double* r) {
+  if (!ctx || which < 0 || which > 2) return fail(ctx, 1, "bad operator index");
+  Shell& S = ctx->shell[which];
+  const size_t n = (size_t)ctx->neq;
+  auto apply = [&](double** dst, const double* v) -> int {
+    std::vector<double> cur(n, 1.0), in(n);
+    if (*dst) HIPC(hipMemcpy(cur.data(), *dst, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (ctx->flags & HR_DEVICE_PTRS) HIPC(hipMemcpy(in.data(), v, n * sizeof(double), hipMemcpyDeviceToHost));
+    else std::memcpy(in.data(), v, n * sizeof(double));
+    for (size_t e = 0; e < n; ++e) cur[e] = cur[e] * in[e];
+    if (!*dst) if (int rc = dalloc(ctx, dst, n)) return rc;
+    HIPC(hipMemcpy(*dst, cur.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (l) if (int rc = apply(&S.l, l)) return rc;
+  if (r) if (int rc = apply(&S.r, r)) return rc;
+  return 0;
+}
+
+int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const double* y, double* z) {
+  if (int rc = check_ready(ctx)) return rc;
+  if (which < 0 || which > 2) return fail(ctx, 1, "bad operator index %d", which);
+  if (!x || !y || !z) return fail(ctx, 1, "null vector");
+  const Shell& S = ctx->shell[which];
+  const long long neq = ctx->neq;
+  const bool dev = ctx->flags & HR_DEVICE_PTRS;
+  double *dx = nullptr, *dy = nullptr, *dz = nullptr;
+  if (dev) { dx = (double*)x; dy = (double*)y; dz = z; }
+  else {
+    if (int rc = dalloc(ctx, &dx, neq)) return rc;
+    if (int rc = dalloc(ctx, &dy, neq)) return rc;
+    dz = dy;
+    HIPC(hipMemcpyAsync(dx, x, neq * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(dy, y, neq * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  // A_eff x = s l (A (r x)) ; A_eff^T x = s r (A^T (l x))
+  const double* pre = transpose ? S.l : S.r;
+  const double* post = transpose ? S.r : S.l;
+  HIPC(launch_gather_eq(neq, ctx->d_eq_slot, dx, pre, ctx->tmp, ctx->stream));
+  StageArgs A = base_args(ctx);
+  A.stage = 0;
+  A.set_mask = which == HR_G ? 1 : (which == HR_K ? 2 : 4);
+  A.transpose = transpose;
+  A.in[0] = ctx->tmp;
+  A.out = ctx->tmp2;
+  if (int rc = launch_stage(ctx, A)) return rc;
+  HIPC(launch_scatter_eq(neq, ctx->d_eq_slot, ctx->tmp2, dy, S.s, post, dz, ctx->stream));
+  if (!dev) {
+    HIPC(hipMemcpyAsync(z, dz, neq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+  }
+  return 0;
+}
+
+int hr_set_tiling(hr_ctx* ctx, int tz) {
+  if (!ctx) return 1;
+  if (tz < 0) return fail(ctx, 1, "bad tiling");
+  Geo& g = ctx->g;
+  g.tz = tz ? tz : DEFAULT_TZ;
+  g.tiles[2] = (g.nez + g.tz - 1) / g.tz;
+  // per-tile source lists of an earlier forward no longer match the tiles
+  if (ctx->d_src_tile_ptr) { (void)hipFree(ctx->d_src_tile_ptr); ctx->d_src_tile_ptr = nullptr; }
+  ctx->f_nstep = 0;
+  ctx->f_history = HR_HISTORY_NONE;
+  return 0;
+}
+
+int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, const long long* src_eq,
+               const double* src_val, int nrec, const long long* rec_eq, double* traces, int history, int segment) {
+  if (int rc = check_ready(ctx)) return rc;
+  if (nstep < 1 || !(dt > 0.0)) return fail(ctx, 1, "nstep must be >= 1 and dt > 0");
+  if (!ftab) return fail(ctx, 1, "null time-function table");
+  if (nsrc < 0 || nrec < 0) return fail(ctx, 1, "negative counts");
+  if (history < 0 || history > 2) return fail(ctx, 1, "bad history mode");
+  if (history == HR_HISTORY_CHECKPOINT && segment < 1) return fail(ctx, 1, "checkpoint segment must be >= 1");
+  const bool dev = ctx->flags & HR_DEVICE_PTRS;
+  // host copies of the small index arrays
+  std::vector<long long> seq(nsrc), req(nrec);
+  std::vector<double> sval(nsrc);
+  if (nsrc) {
+    if (dev) {
+      HIPC(hipMemcpy(seq.data(), src_eq, nsrc * sizeof(long long), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(sval.data(), src_val, nsrc * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      std::memcpy(seq.data(), src_eq, nsrc * sizeof(long long));
+      std::memcpy(sval.data(), src_val, nsrc * sizeof(double));
+    }
+  }
+  if (nrec) {
+    if (dev) HIPC(hipMemcpy(req.data(), rec_eq, nrec * sizeof(long long), hipMemcpyDeviceToHost));
+    else std::memcpy(req.data(), rec_eq, nrec * sizeof(long long));
+  }
+  for (int q = 0; q < nsrc; ++q)
+    if (seq[q] < 0 || seq[q] >= ctx->neq) return fail(ctx, 1, "source equation %lld out of range", seq[q]);
+  for (int q = 0; q < nrec; ++q) {
+    if (req[q] < 0 || req[q] >= ctx->neq) return fail(ctx, 1, "receiver equation %lld out of range", req[q]);
+    if (ctx->eq_slot[req[q]] / ctx->g.NN >= 3) return fail(ctx, 1, "receiver equation %lld is not a displacement", req[q]);
+  }
+  ctx->ftab.resize(3 * (size_t)nstep);
+  if (dev) HIPC(hipMemcpy(ctx->ftab.data(), ftab, 3 * (size_t)nstep * sizeof(double), hipMemcpyDeviceToHost));
+  else std::memcpy(ctx->ftab.data(), ftab, 3 * (size_t)nstep * sizeof(double));
+  // sources grouped by tile
+  if (nsrc) {
+    for (int q = 0; q < nsrc; ++q)
+      if (ctx->eq_slot[seq[q]] / ctx->g.NN >= 3) return fail(ctx, 1, "source equation %lld is not a displacement", seq[q]);
+    SrcBuild sb = build_src(ctx, nsrc, seq.data());
+    std::vector<double> v(nsrc);
+    for (int q = 0; q < nsrc; ++q) v[q] = sval[sb.perm[q]];
+    if (int rc = upload(ctx, &ctx->d_src_tile_ptr, sb.ptr)) return rc;
+    if (int rc = upload(ctx, &ctx->d_src_node, sb.node)) return rc;
+    if (int rc = upload(ctx, &ctx->d_src_comp, sb.comp)) return rc;
+    if (int rc = upload(ctx, &ctx->d_src_m, sb.m)) return rc;
+    if (int rc = upload(ctx, &ctx->d_src_val, v)) return rc;
+  } else if (ctx->d_src_tile_ptr) {
+    (void)hipFree(ctx->d_src_tile_ptr);
+    ctx->d_src_tile_ptr = nullptr;
+  }
+  // receivers
+  ctx->f_nstep = nstep; ctx->f_dt = dt; ctx->f_nrec = nrec; ctx->f_history = history; ctx->f_segment = segment;
+  ctx->rec_slot_h.resize(nrec);
+  for (int q = 0; q < nrec; ++q) ctx->rec_slot_h[q] = ctx->eq_slot[req[q]];
+  if (int rc = upload(ctx, &ctx->d_rec_slot, ctx->rec_slot_h)) return rc;
+  const size_t ntr = (size_t)(nstep + 1) * (nrec > 0 ? nrec : 1);
+  if (ctx->traces_cap < ntr) {
+    if (int rc = dalloc(ctx, &ctx->d_traces, ntr)) return rc;
+    ctx->traces_cap = ntr;
+  }
+  // history storage
+  const size_t slice = 3 * (size_t)ctx->nbox;
+  for (double* p : ctx->ckpt) (void)hipFree(p);
+  ctx->ckpt.clear();
+  if (history == HR_HISTORY_STORE) {
+    const size_t need = (size_t)(nstep + 1) * slice;
+    if (ctx->hist_cap < need) {
+      if (int rc = dalloc(ctx, &ctx->d_hist, need)) return rc;
+      ctx->hist_cap = need;
+    }
+  }
+  if (int rc = zero_state(ctx)) return rc;
+  const double sgn[3] = {1.0, 1.0, 1.0};
+  const size_t nf = (size_t)NCOMP * ctx->g.NN;
+  for (int n = 0; n < nstep; ++n) {
+    if (int rc = record_step(ctx, n)) return rc;
+    if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist + (size_t)n * slice)) return rc;
+    if (history == HR_HISTORY_CHECKPOINT && n % segment == 0) {
+      double* cp = nullptr;
+      if (int rc = dalloc(ctx, &cp, 3 * nf)) return rc;
+      ctx->ckpt.push_back(cp);
+      for (int a = 0; a < 3; ++a)
+        HIPC(hipMemcpyAsync(cp + a * nf, ctx->x[a], nf * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    Source s4[4];
+    forward_sources(ctx, n, s4);
+    if (int rc = rk4_step(ctx, dt, +1, 0, sgn, s4)) return rc;
+  }
+  if (int rc = record_step(ctx, nstep)) return rc;
+  if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist + (size_t)nstep * slice)) return rc;
+  if (traces && nrec) {
+    HIPC(hipMemcpyAsync(traces, ctx->d_traces, (size_t)(nstep + 1) * nrec * sizeof(double),
+                        dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_meas, double* misfit) {
+  if (int rc = check_ready(ctx)) return rc;
+  if (nstep != ctx->f_nstep || nrec != ctx->f_nrec) return fail(ctx, 1, "nstep/nrec differ from the last forward");
+  if (!dis_meas || !misfit) return fail(ctx, 1, "null argument");
+  const bool dev = ctx->flags & HR_DEVICE_PTRS;
+  const size_t nobs = (size_t)(nstep + 1) * nrec;
+  const double* dobs = dis_meas;
+  if (!dev) {
+    if (ctx->obs_cap < nobs) {
+      if (int rc = dalloc(ctx, &ctx->d_obs, nobs)) return rc;
+      ctx->obs_cap = nobs;
+    }
+    HIPC(hipMemcpyAsync(ctx->d_obs, dis_meas, nobs * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    dobs = ctx->d_obs;
+  }
+  if (int rc = dalloc(ctx, &ctx->d_part, nstep + 1)) return rc;
+  if (!ctx->d_scalar) if (int rc = dalloc(ctx, &ctx->d_scalar, 1)) return rc;
+  HIPC(launch_misfit(nstep, nrec, ctx->d_traces, dobs, dt, ctx->d_part, ctx->d_scalar, ctx->stream));
+  HIPC(hipMemcpyAsync(misfit, ctx->d_scalar, sizeof(double), dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                      ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long long* rec_eq, const double* dis_meas,
+                        double* g_lambda, double* g_mu, double* misfit) {
+  if (int rc = check_ready(ctx)) return rc;
+  if (nstep != ctx->f_nstep || nrec != ctx->f_nrec || dt != ctx->f_dt)
+    return fail(ctx, 1, "nstep/dt/nrec differ from the last hr_forward");
+  if (ctx->f_history == HR_HISTORY_NONE) return fail(ctx, 2, "the last hr_forward kept no history");
+  if (!dis_meas || !g_lambda || !g_mu) return fail(ctx, 1, "null argument");
+  const bool dev = ctx->flags & HR_DEVICE_PTRS;
+  std::vector<long long> req(nrec);
+  if (nrec) {
+    if (dev) HIPC(hipMemcpy(req.data(), rec_eq, nrec * sizeof(long long), hipMemcpyDeviceToHost));
+    else std::memcpy(req.data(), rec_eq, nrec * sizeof(long long));
+  }
+  for (int q = 0; q < nrec; ++q)
+    if (ctx->eq_slot[req[q]] != ctx->rec_slot_h[q]) return fail(ctx, 1, "receivers differ from the last hr_forward");
+  const size_t nobs = (size_t)(nstep + 1) * nrec;
+  if (ctx->obs_cap < nobs) {
+    if (int rc = dalloc(ctx, &ctx->d_obs, nobs)) return rc;
+    ctx->obs_cap = nobs;
+  }
+  HIPC(hipMemcpyAsync(ctx->d_obs, dis_meas, nobs * sizeof(double), dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                      ctx->stream));
+  // adjoint source entries grouped by tile; traces/obs permuted into that order
+  SrcBuild sb = build_src(ctx, nrec, req.data());
+  if (int rc = upload(ctx, &ctx->d_rec_tile_ptr, sb.ptr)) return rc;
+  if (int rc = upload(ctx, &ctx->d_rec_node, sb.node)) return rc;
+  if (int rc = upload(ctx, &ctx->d_rec_comp, sb.comp)) return rc;
+  if (int rc = upload(ctx, &ctx->d_rec_m, sb.m)) return rc;
+  // receiver columns of traces / observations in source-entry order (device gather)
+  if (int rc = upload(ctx, &ctx->d_rec_perm, sb.perm)) return rc;
+  if (ctx->sorted_cap < nobs) {
+    if (int rc = dalloc(ctx, &ctx->d_tr_sorted, nobs)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_obs_sorted, nobs)) return rc;
+    ctx->sorted_cap = nobs;
+  }
+  if (nrec) {
+    HIPC(launch_permute_cols(nstep + 1, nrec, ctx->d_rec_perm, ctx->d_traces, ctx->d_tr_sorted, ctx->stream));
+    HIPC(launch_permute_cols(nstep + 1, nrec, ctx->d_rec_perm, ctx->d_obs, ctx->d_obs_sorted, ctx->stream));
+  }
+  // gradient accumulators on the RD box
+  if (int rc = dalloc(ctx, &ctx->d_gl, ctx->nbox)) return rc;
+  if (int rc = dalloc(ctx, &ctx->d_gm, ctx->nbox)) return rc;
+  HIPC(hipMemsetAsync(ctx->d_gl, 0, ctx->nbox * sizeof(double), ctx->stream));
+  HIPC(hipMemsetAsync(ctx->d_gm, 0, ctx->nbox * sizeof(double), ctx->stream));
+  const size_t slice = 3 * (size_t)ctx->nbox;
+  const int seg = ctx->f_segment;
+  if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
+    const size_t need = (size_t)(seg + 1) * slice;
+    if (ctx->seg_cap < need) {
+      if (int rc = dalloc(ctx, &ctx->d_seg, need)) return rc;
+      ctx->seg_cap = need;
+    }
+  }
+  const double sgn[3] = {-1.0, 1.0, -1.0};
+  const size_t nf = (size_t)NCOMP * ctx->g.NN;
+  long long recomputed = 0;
+  int seg_lo = -1;                       // history slices [seg_lo, seg_lo + seg] live in d_seg
+  // adjoint state starts from rest; the forward-state buffers are reused
+  if (int rc = zero_state(ctx)) return rc;
+  // keep the adjoint state apart from the recomputation state: use tmp buffers
+  double* adj[3];
+  double* adjy[3];
+  std::vector<double*> extra;
+  if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
+    for (int a = 0; a < 6; ++a) {
+      double* p = nullptr;
+      if (int rc = dalloc(ctx, &p, nf)) return rc;
+      HIPC(hipMemsetAsync(p, 0, nf * sizeof(double), ctx->stream));
+      extra.push_back(p);
+    }
+    for (int a = 0; a < 3; ++a) { adj[a] = extra[a]; adjy[a] = extra[3 + a]; }
+  } else {
+    for (int a = 0; a < 3; ++a) { adj[a] = ctx->x[a]; adjy[a] = ctx->y[a]; }
+  }
+  int status = 0;
+  for (int n = nstep; n >= 1 && !status; --n) {
+    const double* uh = nullptr;
+    if (n <= nstep - 1) {
+      if (ctx->f_history == HR_HISTORY_STORE) uh = ctx->d_hist + (size_t)n * slice;
+      else {
+        if (seg_lo < 0 || n < seg_lo || n > seg_lo + seg) {
+          // recompute the segment containing n: restore checkpoint, step forward
+          const int c = (n - 1) / seg;       // slices needed: n .. ; segment start c*seg
+          const int s0 = c * seg, s1 = std::min(s0 + seg, nstep);
+          double* cp = ctx->ckpt[c];
+          double* sx[3] = {ctx->x[0], ctx->x[1], ctx->x[2]};
+          double* sy[3] = {ctx->y[0], ctx->y[1], ctx->y[2]};
+          for (int a = 0; a < 3; ++a) {
+            ctx->x[a] = sx[a]; ctx->y[a] = sy[a];
+            HIPC(hipMemcpyAsync(ctx->x[a], cp + a * nf, nf * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+          }
+          if (int rc = run_forward_steps(ctx, s0, s1, ctx->d_seg)) { status = rc; break; }
+          recomputed += s1 - s0;
+          seg_lo = s0;
+        }
+        uh = ctx->d_seg + (size_t)(n - seg_lo) * slice;
+      }
+      HIPC(launch_gradient(ctx->g, ctx->box0, ctx->boxn, uh, adj[1], dt, ctx->d_gl, ctx->d_gm, ctx->stream));
+    }
+    // misfit sources at t_n (stage 1), t_{n-1} (stage 4), mean (stages 2, 3)
+    Source s4[4];
+    for (int s = 0; s < 4; ++s) {
+      std::memset(&s4[s], 0, sizeof(Source));
+      if (!nrec) continue;
+      s4[s].tile_ptr = ctx->d_rec_tile_ptr;
+      s4[s].node = ctx->d_rec_node;
+      s4[s].comp = ctx->d_rec_comp;
+      s4[s].m = ctx->d_rec_m;
+      const double* tn = ctx->d_tr_sorted + (size_t)n * nrec;
+      const double* on = ctx->d_obs_sorted + (size_t)n * nrec;
+      const double* tp = ctx->d_tr_sorted + (size_t)(n - 1) * nrec;
+      const double* op = ctx->d_obs_sorted + (size_t)(n - 1) * nrec;
+      if (s == 0) { s4[s].a = tn; s4[s].a2 = on; s4[s].ca = 1.0; }
+      else if (s == 3) { s4[s].a = tp; s4[s].a2 = op; s4[s].ca = 1.0; }
+      else { s4[s].a = tn; s4[s].a2 = on; s4[s].ca = 0.5; s4[s].b = tp; s4[s].b2 = op; s4[s].cb = 0.5; }
+    }
+    // adjoint step on adj -> adjy
+    double* fx[3] = {ctx->x[0], ctx->x[1], ctx->x[2]};
+    double* fy[3] = {ctx->y[0], ctx->y[1], ctx->y[2]};
+    for (int a = 0; a < 3; ++a) { ctx->x[a] = adj[a]; ctx->y[a] = adjy[a]; }
+    if (int rc = rk4_step(ctx, dt, -1, 1, sgn, s4)) { status = rc; }
+    for (int a = 0; a < 3; ++a) { adj[a] = ctx->x[a]; adjy[a] = ctx->y[a]; }
+    if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
+      for (int a = 0; a < 3; ++a) { ctx->x[a] = fx[a]; ctx->y[a] = fy[a]; }
+    }
+  }
+  if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
+    HIPC(hipStreamSynchronize(ctx->stream));
+    for (double* p : extra) (void)hipFree(p);
+  }
+  if (status) return status;
+  ctx->recompute = 1.0 + (double)recomputed / (double)nstep;
+  // nodal outputs: zero, then -g on the box
+  double *dgl = g_lambda, *dgm = g_mu;
+  const size_t NN = (size_t)ctx->g.NN;
+  if (!dev) {
+    dgl = ctx->tmp;
+    dgm = ctx->tmp2;
+  }
+  HIPC(hipMemsetAsync(dgl, 0, NN * sizeof(double), ctx->stream));
+  HIPC(hipMemsetAsync(dgm, 0, NN * sizeof(double), ctx->stream));
+  HIPC(launch_gradient_finish(ctx->g, ctx->box0, ctx->boxn, ctx->d_gl, ctx->d_gm, dgl, dgm, ctx->stream));
+  if (!dev) {
+    HIPC(hipMemcpyAsync(g_lambda, dgl, NN * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipMemcpyAsync(g_mu, dgm, NN * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(hipMemsetAsync(ctx->tmp, 0, NN * sizeof(double) * NCOMP, ctx->stream));
+    HIPC(hipMemsetAsync(ctx->tmp2, 0, NN * sizeof(double) * NCOMP, ctx->stream));
+  }
+  if (misfit) {
+    if (int rc = dalloc(ctx, &ctx->d_part, nstep + 1)) return rc;
+    if (!ctx->d_scalar) if (int rc = dalloc(ctx, &ctx->d_scalar, 1)) return rc;
+    HIPC(launch_misfit(nstep, nrec, ctx->d_traces, ctx->d_obs, dt, ctx->d_part, ctx->d_scalar, ctx->stream));
+    HIPC(hipMemcpyAsync(misfit, ctx->d_scalar, sizeof(double), dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                        ctx->stream));
+  }
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+double hr_recompute_ratio(const hr_ctx* ctx) { return ctx ? ctx->recompute : 0.0; }
+
+int hr_set_profiling(hr_ctx* ctx, int mode) {
+  if (!ctx) return 1;
+  if (int rc = drain_events(ctx)) return rc;
+  if (mode == 2) {
+    for (int s = 0; s < 5; ++s) { ctx->prof_n[s] = 0; ctx->prof_ms[s] = 0.0; }
+    return 0;
+  }
+  ctx->profiling = mode ? 1 : 0;
+  return 0;
+}
+
+int hr_get_kernel_stats(hr_ctx* ctx, long long* launches, double* ms) {
+  if (!ctx || !launches || !ms) return fail(ctx, 1, "null argument");
+  if (int rc = drain_events(ctx)) return rc;
+  for (int s = 0; s < 5; ++s) { launches[s] = ctx->prof_n[s]; ms[s] = ctx->prof_ms[s]; }
+  return 0;
+}
+
+}  // extern "C"

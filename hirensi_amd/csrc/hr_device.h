@@ -1,0 +1,80 @@
+// Device-side data structures shared by the HIP kernels and the host API.
+#pragma once
+#include <cstdint>
+
+namespace hr {
+
+constexpr int NCOMP = 9;           // u_x u_y u_z s_xx s_yy s_zz s_xy s_xz s_yz
+
+// Structured box geometry and the per-element constants (all elements of the
+// box share one geometry: edges hx, hy, hz).
+struct Geo {
+  int nex, ney, nez;               // elements per axis (PML included)
+  int NX, NY, NZ;                  // lattice nodes per axis (2*ne + 1)
+  long long NN;                    // lattice nodes
+  int rlo[3], rhi[3];              // regular-domain element box [lo, hi)
+  int top_rule;                    // 1: original top PML nodes lose s_zz, s_xz, s_yz
+  int tiles[3];                    // stage-kernel tiles per axis
+  int tz;                          // elements per z-chunk of a tile
+  double dxi[3];                   // dxi/dx = DJI(k,k) of the affine element
+  double wq[27];                   // GLL weight * det J at local point a + 3b + 9c
+  double pml[24];                  // PML_PARAM(6,4), column-major
+};
+
+// Per-node arrays on the lattice (length NN).
+struct NodeData {
+  const double *xc, *yc, *zc;      // lattice coordinates per axis
+  const double *lam, *mu;          // nodal Lame parameters
+  const double *minv_u;            // 1 / lumped displacement mass
+  const double *rwp;               // sum over PML elements of rho_e * W_e at the node
+  const double *wps;               // sum over PML elements of W_e at the node
+};
+
+// Sparse per-stage source: entries grouped by stage tile.
+struct Source {
+  const int* tile_ptr;             // (ntiles + 1)
+  const long long* node;           // lattice node
+  const int* comp;                 // component 0..2
+  const double* m;                 // M^-1 of that DOF
+  const double *a, *a2;            // value_j = ca*((a[j] - a2[j]) * m[j]) + cb*((b[j] - b2[j]) * m[j])
+  const double *b, *b2;
+  double ca, cb;
+};
+
+struct StageArgs {
+  Geo g;
+  NodeData nd;
+  const double* in[6];             // stage: x1 x2 x3 g1 g2 g3; apply: in[0] = x
+  int stage;                       // 0 = apply (raw operator), 1..4 = RK4 stage
+  int set_mask;                    // bit0: G on v1, bit1: K on v2, bit2: C on v3
+  int transpose;                   // 1: apply the transposed operator
+  double sgn[3];                   // multipliers on v1, v2, v3
+  double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed)
+  double* out;                     // stage 1-3: g_s; apply: raw result
+  double* xnew[3];                 // stage 4: updated x1, x2, x3
+  Source src;
+  int nsrc_tiles;                  // 0: no source
+};
+
+}  // namespace hr
+
+#include <hip/hip_runtime.h>
+namespace hr {
+// Host launchers (defined with the kernels).
+hipError_t launch_stage(const StageArgs& A, int nblocks, hipStream_t s);
+hipError_t launch_gather_eq(long long neq, const long long* slot, const double* x, const double* r, double* out,
+                            hipStream_t s);
+hipError_t launch_scatter_eq(long long neq, const long long* slot, const double* raw, const double* y, double sc,
+                             const double* l, double* z, hipStream_t s);
+hipError_t launch_record(int nrec, const long long* slot, const double* x2, double* row, hipStream_t s);
+hipError_t launch_store_box(const Geo& g, const int* box0, const int* boxn, const double* x2, double* hist,
+                            hipStream_t s);
+hipError_t launch_gradient(const Geo& g, const int* box0, const int* boxn, const double* uh, const double* p2,
+                           double dt, double* glam, double* gmu, hipStream_t s);
+hipError_t launch_gradient_finish(const Geo& g, const int* box0, const int* boxn, const double* glb,
+                                  const double* gmb, double* gl, double* gm, hipStream_t s);
+hipError_t launch_permute_cols(long long nrows, int ncols, const long long* perm, const double* src, double* dst,
+                               hipStream_t s);
+hipError_t launch_misfit(int nstep, int nrec, const double* tr, const double* ob, double dt, double* part,
+                         double* out, hipStream_t s);
+}  // namespace hr

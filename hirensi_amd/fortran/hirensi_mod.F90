@@ -1,0 +1,186 @@
+! ISO_C_BINDING interface to the C ABI of include/hirensi.h, for the
+! reference's Fortran driver (step_4_3D_Forward_and_Inverse/).  USE HIRENSI
+! next to USE PARAMETERS; the calls replace the assembled-matrix path:
+!   ASSEM_*_PETSC_3D + MatAssembly      -> hr_ctx_create, hr_set_pml, hr_set_material
+!   EXPLICIT_RK4_PETSC_PML_3D           -> hr_forward
+!   *_Adjoint_OtD + *_Control_OtD       -> hr_adjoint_gradient
+!   MatMultAdd / MatMultTransposeAdd    -> hr_apply (or the MatShell adapter)
+module hirensi
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: HR_K = 0, HR_C = 1, HR_G = 2
+  integer(c_int), parameter :: HR_DEVICE_PTRS = 1
+  integer(c_int), parameter :: HR_HISTORY_NONE = 0, HR_HISTORY_STORE = 1, HR_HISTORY_CHECKPOINT = 2
+
+  type, bind(C) :: hr_mesh_desc
+    integer(c_int) :: nex, ney, nez
+    integer(c_int) :: npml(5)            ! x-, x+, y-, y+, z-
+    real(c_double) :: h(3)
+    integer(c_int) :: top_stress_bc
+    integer(c_int) :: flags
+  end type hr_mesh_desc
+
+  interface
+    integer(c_int) function hr_ctx_create(desc, ctx) bind(C, name="hr_ctx_create")
+      import :: c_int, c_ptr, hr_mesh_desc
+      type(hr_mesh_desc), intent(in) :: desc
+      type(c_ptr), intent(out) :: ctx
+    end function hr_ctx_create
+
+    integer(c_int) function hr_ctx_destroy(ctx) bind(C, name="hr_ctx_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function hr_ctx_destroy
+
+    type(c_ptr) function hr_last_error(ctx) bind(C, name="hr_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function hr_last_error
+
+    integer(c_int) function hr_get_sizes(ctx, neq, nnode, nstore) bind(C, name="hr_get_sizes")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), intent(out) :: neq, nnode, nstore
+    end function hr_get_sizes
+
+    integer(c_int) function hr_get_store_eq(ctx, store_eq) bind(C, name="hr_get_store_eq")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), intent(out) :: store_eq(*)
+    end function hr_get_store_eq
+
+    integer(c_int) function hr_set_pml(ctx, pml_param) bind(C, name="hr_set_pml")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: pml_param(6, 4)
+    end function hr_set_pml
+
+    integer(c_int) function hr_set_material(ctx, lam, mu, rho_elem) bind(C, name="hr_set_material")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: lam(*), mu(*), rho_elem(*)
+    end function hr_set_material
+
+    integer(c_int) function hr_get_diag_m(ctx, diag_m) bind(C, name="hr_get_diag_m")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: diag_m(*)
+    end function hr_get_diag_m
+
+    integer(c_int) function hr_scale(ctx, which, s) bind(C, name="hr_scale")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
+      real(c_double), value :: s
+    end function hr_scale
+
+    integer(c_int) function hr_diagonal_scale(ctx, which, l, r) bind(C, name="hr_diagonal_scale")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
+      type(c_ptr), value :: l, r          ! c_loc(vector) or c_null_ptr
+    end function hr_diagonal_scale
+
+    integer(c_int) function hr_reset_scaling(ctx, which) bind(C, name="hr_reset_scaling")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
+    end function hr_reset_scaling
+
+    integer(c_int) function hr_apply(ctx, which, transpose, x, y, z) bind(C, name="hr_apply")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which, transpose
+      real(c_double), intent(in) :: x(*), y(*)
+      real(c_double), intent(inout) :: z(*)
+    end function hr_apply
+
+    real(c_double) function hr_time_function(t, lcase, fr) bind(C, name="hr_time_function")
+      import :: c_double, c_int
+      real(c_double), value :: t, fr
+      integer(c_int), value :: lcase
+    end function hr_time_function
+
+    integer(c_int) function hr_forward(ctx, nstep, dt, ftab, nsrc, src_eq, src_val, nrec, rec_eq, traces, &
+                                       history, segment) bind(C, name="hr_forward")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nstep, nsrc, nrec, history, segment
+      real(c_double), value :: dt
+      real(c_double), intent(in) :: ftab(3, *), src_val(*)
+      integer(c_long_long), intent(in) :: src_eq(*), rec_eq(*)
+      real(c_double), intent(out) :: traces(*)
+    end function hr_forward
+
+    integer(c_int) function hr_adjoint_gradient(ctx, nstep, dt, nrec, rec_eq, dis_meas, g_lambda, g_mu, misfit) &
+        bind(C, name="hr_adjoint_gradient")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nstep, nrec
+      real(c_double), value :: dt
+      integer(c_long_long), intent(in) :: rec_eq(*)
+      real(c_double), intent(in) :: dis_meas(*)
+      real(c_double), intent(out) :: g_lambda(*), g_mu(*)
+      real(c_double), intent(out) :: misfit
+    end function hr_adjoint_gradient
+
+    integer(c_int) function hr_misfit(ctx, nstep, dt, nrec, dis_meas, misfit) bind(C, name="hr_misfit")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nstep, nrec
+      real(c_double), value :: dt
+      real(c_double), intent(in) :: dis_meas(*)
+      real(c_double), intent(out) :: misfit
+    end function hr_misfit
+
+    real(c_double) function hr_recompute_ratio(ctx) bind(C, name="hr_recompute_ratio")
+      import :: c_double, c_ptr
+      type(c_ptr), value :: ctx
+    end function hr_recompute_ratio
+
+    integer(c_int) function hr_set_tiling(ctx, tz) bind(C, name="hr_set_tiling")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: tz
+    end function hr_set_tiling
+
+    integer(c_int) function hr_set_profiling(ctx, mode) bind(C, name="hr_set_profiling")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+    end function hr_set_profiling
+
+    integer(c_int) function hr_get_kernel_stats(ctx, launches, ms) bind(C, name="hr_get_kernel_stats")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), intent(out) :: launches(5)
+      real(c_double), intent(out) :: ms(5)
+    end function hr_get_kernel_stats
+  end interface
+
+contains
+
+  ! Fortran string of hr_last_error
+  function hr_error_message(ctx) result(msg)
+    type(c_ptr), value :: ctx
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    interface
+      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+        import :: c_ptr, c_size_t
+        type(c_ptr), value :: s
+      end function c_strlen
+    end interface
+    type(c_ptr) :: cp
+    integer :: i, n
+    cp = hr_last_error(ctx)
+    n = int(c_strlen(cp))
+    call c_f_pointer(cp, p, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = p(i)
+    end do
+  end function hr_error_message
+
+end module hirensi

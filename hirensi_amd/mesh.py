@@ -262,14 +262,11 @@ def layered_model(mesh: BoxMesh, vs_layers, rho_layers, interfaces, anomaly=None
     for zi, r in zip(reversed(interfaces), reversed(rho_layers[:-1])):
         rho_z = np.where(zc >= zi, float(r), rho_z)
     rho_el = np.repeat(rho_z, mesh.nex * mesh.ney)
-    rho_node_dummy = 1.0
-    lam, mu, _ = vs_vp_rho_to_lame(vs, vp, rho_node_dummy)
     # lambda, mu use the density of the layer the (mapped) node lies in
     rho_n = np.full(mesh.nnode, float(rho_layers[-1]))
     for zi, r in zip(reversed(interfaces), reversed(rho_layers[:-1])):
         rho_n = np.where(z >= zi, float(r), rho_n)
-    lam = lam * rho_n
-    mu = mu * rho_n
+    lam, mu, _ = vs_vp_rho_to_lame(vs, vp, rho_n)
     return Model(lam, mu, rho_el * 1.0e-6)
 
 

@@ -1,0 +1,79 @@
+"""ctypes binding of ``libhirensi.so`` (the C ABI of ``include/hirensi.h``).
+
+The library is loaded from the package directory; a missing library is an
+error, there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhirensi.so")
+
+HR_DEVICE_PTRS = 1
+HR_K, HR_C, HR_G = 0, 1, 2
+HR_HISTORY_NONE, HR_HISTORY_STORE, HR_HISTORY_CHECKPOINT = 0, 1, 2
+
+EXPORTS = (
+    "hr_ctx_create", "hr_ctx_destroy", "hr_last_error", "hr_get_sizes", "hr_get_store_eq",
+    "hr_set_pml", "hr_set_material", "hr_get_diag_m", "hr_scale", "hr_diagonal_scale",
+    "hr_reset_scaling", "hr_apply", "hr_time_function", "hr_forward", "hr_adjoint_gradient",
+    "hr_misfit", "hr_recompute_ratio", "hr_set_tiling", "hr_set_profiling", "hr_get_kernel_stats",
+)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("nex", C.c_int), ("ney", C.c_int), ("nez", C.c_int),
+        ("npml", C.c_int * 5),
+        ("h", C.c_double * 3),
+        ("top_stress_bc", C.c_int),
+        ("flags", C.c_int),
+    ]
+
+
+class HrError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_vp = C.c_void_p
+_dp = C.c_void_p      # double* passed as raw address (host numpy or device pointer)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HrError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    L.hr_ctx_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(_vp)]
+    L.hr_ctx_destroy.argtypes = [_vp]
+    L.hr_last_error.argtypes = [_vp]
+    L.hr_last_error.restype = C.c_char_p
+    L.hr_get_sizes.argtypes = [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.hr_get_store_eq.argtypes = [_vp, _dp]
+    L.hr_set_pml.argtypes = [_vp, _dp]
+    L.hr_set_material.argtypes = [_vp, _dp, _dp, _dp]
+    L.hr_get_diag_m.argtypes = [_vp, _dp]
+    L.hr_scale.argtypes = [_vp, C.c_int, C.c_double]
+    L.hr_diagonal_scale.argtypes = [_vp, C.c_int, _dp, _dp]
+    L.hr_reset_scaling.argtypes = [_vp, C.c_int]
+    L.hr_apply.argtypes = [_vp, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.hr_time_function.argtypes = [C.c_double, C.c_int, C.c_double]
+    L.hr_time_function.restype = C.c_double
+    L.hr_forward.argtypes = [_vp, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int]
+    L.hr_adjoint_gradient.argtypes = [_vp, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.hr_misfit.argtypes = [_vp, C.c_int, C.c_double, C.c_int, _dp, _dp]
+    L.hr_recompute_ratio.argtypes = [_vp]
+    L.hr_recompute_ratio.restype = C.c_double
+    L.hr_set_tiling.argtypes = [_vp, C.c_int]
+    L.hr_set_profiling.argtypes = [_vp, C.c_int]
+    L.hr_get_kernel_stats.argtypes = [_vp, _dp, _dp]
+    for name in EXPORTS:
+        getattr(L, name).restype = getattr(L, name).restype or C.c_int
+    _lib = L
+    return L

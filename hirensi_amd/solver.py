@@ -1,0 +1,258 @@
+"""Host-side mirror of the reference's explicit solver interface, on top of the
+HIP operator (``libhirensi.so``).
+
+Names follow the reference (step_4_3D_Forward_and_Inverse/):
+
+* :class:`ShellMat` -- the PETSc ``Mat`` calls the solver makes on AK, AC, AG:
+  ``MatMultAdd``, ``MatMultTransposeAdd``, ``MatDiagonalScale``, ``MatScale``,
+  ``MatZeroEntries`` (Solver_PETSc_Explicit.F90:815-824,1005-1034,1222-1229,
+  1323-1352; Program_Main.F90:692-697).
+* :func:`time_function` -- TIME_FUNCTION (IO.F90:541-733).
+* :func:`explicit_rk4_petsc_pml_3d` -- the forward RK4 solve
+  (Solver_PETSc_Explicit.F90:660-1117).
+* :func:`explicit_rk4_petsc_pml_3d_adjoint_otd` -- adjoint, control pass and
+  sign flip (Solver_PETSc_Explicit.F90:1125-1522, Program_Main.F90:400-413).
+* :func:`compute_misfit` -- Compute_Misfit (Solver_PETSc_Inverse.F90:591-650).
+
+Errors raise :class:`HrError` with the library's message, the way the
+reference would stop; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native
+from .native import HrError, HR_K, HR_C, HR_G, HR_HISTORY_NONE, HR_HISTORY_STORE, HR_HISTORY_CHECKPOINT
+
+__all__ = ["Context", "ShellMat", "time_function", "time_table", "explicit_rk4_petsc_pml_3d",
+           "explicit_rk4_petsc_pml_3d_adjoint_otd", "compute_misfit", "HrError",
+           "HR_K", "HR_C", "HR_G", "HR_HISTORY_NONE", "HR_HISTORY_STORE", "HR_HISTORY_CHECKPOINT"]
+
+
+def _ptr(a):
+    """Address of a host numpy array or of a device tensor (``data_ptr``)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    if isinstance(a, int):
+        return a
+    raise TypeError(f"unsupported array type {type(a)!r}")
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+class Context:
+    """One mesh + material on the GPU (``hr_ctx``).
+
+    ``mesh`` is a :class:`hirensi_amd.mesh.BoxMesh`, ``model`` a
+    :class:`hirensi_amd.mesh.Model`.  With ``device_ptrs=True`` every array
+    passed to the methods must be device memory (e.g. torch tensors on cuda).
+    """
+
+    def __init__(self, mesh, model=None, device_ptrs=False, tz=0):
+        self.L = native.load()
+        self.mesh = mesh
+        self.device_ptrs = bool(device_ptrs)
+        d = native.MeshDesc()
+        d.nex, d.ney, d.nez = mesh.nex, mesh.ney, mesh.nez
+        for a in range(5):
+            d.npml[a] = mesh.npml[a]
+        for a in range(3):
+            d.h[a] = mesh.h[a]
+        d.top_stress_bc = 1 if mesh.top_stress_bc == "reference" else 0
+        d.flags = native.HR_DEVICE_PTRS if device_ptrs else 0
+        h = C.c_void_p()
+        rc = self.L.hr_ctx_create(C.byref(d), C.byref(h))
+        self.h = h
+        self._check(rc)
+        if tz:
+            self._check(self.L.hr_set_tiling(self.h, int(tz)))
+        neq, nn, ns = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        self._check(self.L.hr_get_sizes(self.h, C.byref(neq), C.byref(nn), C.byref(ns)))
+        self.neq, self.nnode, self.nstore = neq.value, nn.value, ns.value
+        if self.neq != mesh.neq:
+            raise HrError(f"equation count {self.neq} differs from the mesh model {mesh.neq}")
+        pp = _f64(np.asarray(mesh.pml_param()).T)      # Fortran column-major (6,4)
+        self._pml = pp
+        self._check(self.L.hr_set_pml(self.h, pp.ctypes.data))
+        if model is not None:
+            self.set_material(model)
+
+    # ------------------------------------------------------------------ util
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.L.hr_last_error(self.h).decode() if self.h else "context creation failed"
+            raise HrError(f"hirensi error {rc}: {msg}")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.hr_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _host(self, a, conv):
+        return a if self.device_ptrs else conv(a)
+
+    # --------------------------------------------------------------- set-up
+    def set_material(self, model):
+        if self.device_ptrs:
+            raise HrError("set_material takes host arrays; use set_material_ptrs with device memory")
+        lam, mu, rho = _f64(model.lam), _f64(model.mu), _f64(model.rho)
+        self._check(self.L.hr_set_material(self.h, lam.ctypes.data, mu.ctypes.data, rho.ctypes.data))
+
+    def set_material_ptrs(self, lam, mu, rho):
+        self._check(self.L.hr_set_material(self.h, _ptr(lam), _ptr(mu), _ptr(rho)))
+
+    def store_eq(self):
+        out = np.zeros(self.nstore, dtype=np.int64)
+        if self.device_ptrs:
+            raise HrError("store_eq returns host data; create a host-pointer context")
+        self._check(self.L.hr_get_store_eq(self.h, out.ctypes.data))
+        return out
+
+    def diag_m(self):
+        out = np.zeros(self.neq)
+        if self.device_ptrs:
+            raise HrError("diag_m returns host data; create a host-pointer context")
+        self._check(self.L.hr_get_diag_m(self.h, out.ctypes.data))
+        return out
+
+    def recompute_ratio(self):
+        return self.L.hr_recompute_ratio(self.h)
+
+
+class ShellMat:
+    """PETSc ``Mat`` of one operator (K, C or G) as a matrix-free shell."""
+
+    def __init__(self, ctx: Context, which: int):
+        self.ctx, self.which = ctx, int(which)
+
+    def mult_add(self, x, y, z=None):
+        """MatMultAdd: z = y + A x."""
+        return self._apply(0, x, y, z)
+
+    def mult_transpose_add(self, x, y, z=None):
+        """MatMultTransposeAdd: z = y + A^T x."""
+        return self._apply(1, x, y, z)
+
+    def _apply(self, tr, x, y, z):
+        c = self.ctx
+        if not c.device_ptrs:
+            x, y = _f64(x), _f64(y)
+            if z is None:
+                z = np.empty(c.neq)
+        elif z is None:
+            raise HrError("device shells need an output vector")
+        c._check(c.L.hr_apply(c.h, self.which, tr, _ptr(x), _ptr(y), _ptr(z)))
+        return z
+
+    def scale(self, s):
+        """MatScale."""
+        self.ctx._check(self.ctx.L.hr_scale(self.ctx.h, self.which, float(s)))
+
+    def diagonal_scale(self, l=None, r=None):
+        """MatDiagonalScale(A, l, r); None leaves that side unchanged."""
+        c = self.ctx
+        if not c.device_ptrs:
+            l = None if l is None else _f64(l)
+            r = None if r is None else _f64(r)
+        c._check(c.L.hr_diagonal_scale(c.h, self.which, _ptr(l), _ptr(r)))
+
+    def zero_entries(self):
+        """MatZeroEntries followed by re-assembly: the shell returns to A."""
+        self.ctx._check(self.ctx.L.hr_reset_scaling(self.ctx.h, self.which))
+
+
+def time_function(t, lcase=2, fr=15.0):
+    """TIME_FUNCTION (IO.F90:541-733); lcase 2 = Ricker of central frequency fr."""
+    return native.load().hr_time_function(float(t), int(lcase), float(fr))
+
+
+def time_table(nstep, dt, lcase=2, fr=15.0):
+    """f(T), f(T + dt/2), f(T + dt) with T = ISTEP*dt, for ISTEP = 0..nstep-1
+    (Solver_PETSc_Explicit.F90:853,983-995)."""
+    out = np.empty((nstep, 3))
+    for n in range(nstep):
+        T = float(n) * dt
+        out[n, 0] = time_function(T, lcase, fr)
+        out[n, 1] = time_function(T + 0.5 * dt, lcase, fr)
+        out[n, 2] = time_function(T + dt, lcase, fr)
+    return out
+
+
+def explicit_rk4_petsc_pml_3d(ctx: Context, nstep, dt, ftab, src_eq, src_val, rec_eq, history=HR_HISTORY_STORE,
+                              segment=64, traces=None):
+    """Forward solve from rest; returns the receiver traces, row n = u at the
+    receiver equations at the start of step n (n = 0..nstep)."""
+    nrec = len(rec_eq)
+    if ctx.device_ptrs:
+        if traces is None:
+            raise HrError("device contexts need a traces buffer")
+        fptr, sptr, vptr, rptr = _ptr(ftab), _ptr(src_eq), _ptr(src_val), _ptr(rec_eq)
+        nsrc = len(src_eq)
+    else:
+        ftab = _f64(ftab)
+        if ftab.shape[0] != nstep:
+            raise ValueError("time table must have nstep rows")
+        src_eq, src_val, rec_eq = _i64(src_eq), _f64(src_val), _i64(rec_eq)
+        nsrc = len(src_eq)
+        if traces is None:
+            traces = np.zeros((nstep + 1, nrec))
+        fptr, sptr, vptr, rptr = ftab.ctypes.data, src_eq.ctypes.data, src_val.ctypes.data, rec_eq.ctypes.data
+    ctx._check(ctx.L.hr_forward(ctx.h, int(nstep), float(dt), fptr, nsrc, sptr if nsrc else None,
+                                vptr if nsrc else None, nrec, rptr if nrec else None,
+                                _ptr(traces) if nrec else None, int(history), int(segment)))
+    return traces
+
+
+def explicit_rk4_petsc_pml_3d_adjoint_otd(ctx: Context, nstep, dt, rec_eq, dis_meas, g_lambda=None, g_mu=None,
+                                          with_misfit=True, misfit_out=None):
+    """OtD adjoint + control pass + sign flip; returns (g_lambda, g_mu, misfit)
+    with nodal gradients (zero outside the regular-domain node box)."""
+    nrec = len(rec_eq)
+    if ctx.device_ptrs:
+        if g_lambda is None or g_mu is None:
+            raise HrError("device contexts need gradient buffers")
+        mptr = _ptr(misfit_out) if with_misfit else None
+        ctx._check(ctx.L.hr_adjoint_gradient(ctx.h, int(nstep), float(dt), nrec, _ptr(rec_eq), _ptr(dis_meas),
+                                             _ptr(g_lambda), _ptr(g_mu), mptr))
+        return g_lambda, g_mu, misfit_out
+    rec_eq, dis_meas = _i64(rec_eq), _f64(dis_meas)
+    if g_lambda is None:
+        g_lambda = np.zeros(ctx.nnode)
+    if g_mu is None:
+        g_mu = np.zeros(ctx.nnode)
+    m = np.zeros(1)
+    ctx._check(ctx.L.hr_adjoint_gradient(ctx.h, int(nstep), float(dt), nrec, rec_eq.ctypes.data,
+                                         dis_meas.ctypes.data, g_lambda.ctypes.data, g_mu.ctypes.data,
+                                         m.ctypes.data if with_misfit else None))
+    return g_lambda, g_mu, float(m[0])
+
+
+def compute_misfit(ctx: Context, nstep, dt, dis_meas):
+    """Compute_Misfit of the traces of the last forward solve."""
+    if ctx.device_ptrs:
+        raise HrError("compute_misfit returns a host scalar; use a host-pointer context")
+    dis_meas = _f64(dis_meas)
+    nrec = dis_meas.shape[1]
+    m = np.zeros(1)
+    ctx._check(ctx.L.hr_misfit(ctx.h, int(nstep), float(dt), nrec, dis_meas.ctypes.data, m.ctypes.data))
+    return float(m[0])

@@ -1,0 +1,134 @@
+/*
+ * hirensi.h - C ABI of the MI355X explicit elastic forward / adjoint operator.
+ *
+ * Replaces the assembled-matrix path of HiReNSI's explicit solver
+ * (step_4_3D_Forward_and_Inverse/, PETSc 3.4 Mat/Vec API):
+ *   - the matrices AK, AC, AG assembled by ASSEM_REGULAR_DOMAIN_PETSC_3D and
+ *     ASSEM_PML_PETSC_3D (Assemble_PETSc_3D.F90:6-436) and their products
+ *     MatMultAdd / MatMultTransposeAdd (Solver_PETSc_Explicit.F90:1005-1034,
+ *     1323-1352), with the in-place MatDiagonalScale / MatScale of
+ *     Solver_PETSc_Explicit.F90:815-824,1222-1229;
+ *   - the RK4 forward loop EXPLICIT_RK4_PETSC_PML_3D
+ *     (Solver_PETSc_Explicit.F90:660-1117, called at Solve_Forward.F90:117);
+ *   - the OtD adjoint EXPLICIT_RK4_PETSC_PML_3D_Adjoint_OtD
+ *     (Solver_PETSc_Explicit.F90:1125-1405), the control pass
+ *     EXPLICIT_RK4_PETSC_PML_3D_Control_OtD + Update_g
+ *     (Solver_PETSc_Explicit.F90:1413-1522, Assemble_PETSc_3D.F90:556-661)
+ *     and the sign flip of Program_Main.F90:404-413;
+ *   - Compute_Misfit (Solver_PETSc_Inverse.F90:591-650).
+ *
+ * Conventions
+ *   - Equation numbers are 0-based, node-major with the active DOFs of a node
+ *     interleaved (u_x u_y u_z s_xx s_yy s_zz s_xy s_xz s_yz), nodes in
+ *     lattice order n = i + NX*(j + NY*k) of the structured box (NX = 2*nex+1).
+ *   - All calls return 0 on success, a positive code otherwise; the message
+ *     is available from hr_last_error().  No call throws.
+ *   - Pointers are host pointers unless the context was created with
+ *     HR_DEVICE_PTRS, in which case every array argument is device memory.
+ *   - Calls on one context are stream-ordered and not re-entrant.
+ */
+#ifndef HIRENSI_H
+#define HIRENSI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hr_ctx hr_ctx;
+
+#define HR_DEVICE_PTRS 1
+
+/* which operator */
+#define HR_K 0
+#define HR_C 1
+#define HR_G 2
+
+/* history handling of the forward wavefield for the adjoint */
+#define HR_HISTORY_NONE 0
+#define HR_HISTORY_STORE 1        /* keep u on the regular-domain box for every step */
+#define HR_HISTORY_CHECKPOINT 2   /* keep the state every `segment` steps, recompute */
+
+/* Structured box mesh: the ANSYS box + PML of the reference pre-processing
+ * (step_1_Ansys_Mesh_Generator, step_2_20_To_27/Main.F90:447-521). */
+typedef struct hr_mesh_desc {
+  int nex, ney, nez;        /* elements per axis, PML included */
+  int npml[5];              /* PML element layers: x-, x+, y-, y+, z- (top is free) */
+  double h[3];              /* element edge lengths */
+  int top_stress_bc;        /* 1: s_zz, s_xz, s_yz fixed at original top PML nodes */
+  int flags;                /* HR_DEVICE_PTRS */
+} hr_mesh_desc;
+
+/* Context and mesh (replaces MatCreateAIJ, PETSc_Global_Objects.F90:46-51). */
+int hr_ctx_create(const hr_mesh_desc* desc, hr_ctx** out);
+int hr_ctx_destroy(hr_ctx* ctx);
+const char* hr_last_error(const hr_ctx* ctx);
+int hr_get_sizes(hr_ctx* ctx, long long* neq, long long* nnode, long long* nstore);
+/* stored set (U_Store_Numbers_Global, PIC_Inversion_Data_Structure.F90:276-301):
+ * equation numbers of the u DOFs of the regular-domain node box, in the order
+ * the history is kept (component-major, box lattice order) */
+int hr_get_store_eq(hr_ctx* ctx, long long* store_eq);
+
+/* PML_PARAM(6,4), Fortran column-major: [alpha0 beta0 L x0] for faces
+ * x+ x- y+ y- z+ z- (Adjustments.F90:53-84, PML.F90:325).  Always host
+ * memory (a 24-value parameter block), also under HR_DEVICE_PTRS. */
+int hr_set_pml(hr_ctx* ctx, const double* pml_param);
+
+/* Nodal lambda and mu (PMat_Lambda, PMat_Mu) and per-element density
+ * (PMAT(MTEL,3)), element order e = ex + nex*(ey + ney*ez).  Replaces the
+ * element loops of ASSEM_REGULAR_DOMAIN_PETSC_3D / ASSEM_PML_PETSC_3D. */
+int hr_set_material(hr_ctx* ctx, const double* lam, const double* mu, const double* rho_elem);
+
+/* Lumped mass DIAG_M (Solve_Forward.F90:96-97), reference numbering. */
+int hr_get_diag_m(hr_ctx* ctx, double* diag_m);
+
+/* MatShell state of operator `which`: A_eff = s * diag(l) * A * diag(r).
+ * hr_scale multiplies s (MatScale), hr_diagonal_scale multiplies l and r
+ * elementwise (MatDiagonalScale; NULL leaves a side unchanged),
+ * hr_reset_scaling restores s = 1, l = r = 1 (MatZeroEntries + reassembly). */
+int hr_scale(hr_ctx* ctx, int which, double s);
+int hr_diagonal_scale(hr_ctx* ctx, int which, const double* l, const double* r);
+int hr_reset_scaling(hr_ctx* ctx, int which);
+
+/* z = y + A_eff x (MatMultAdd) or z = y + A_eff^T x (MatMultTransposeAdd).
+ * x, y, z have neq entries; z may alias y. */
+int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const double* y, double* z);
+
+/* TIME_FUNCTION (IO.F90:541-733): lcase 1 sine, 2 Ricker, 3 Gaussian. */
+double hr_time_function(double t, int lcase, double fr);
+
+/* Forward RK4 from rest (EXPLICIT_RK4_PETSC_PML_3D).
+ *   ftab[3*n + {0,1,2}] = f(n dt), f(n dt + dt/2), f(n dt + dt) for n < nstep
+ *   B = sum of src_val at src_eq (force vector before M^-1 scaling)
+ *   traces[(nstep+1) x nrec]: u at rec_eq at the start of steps 0..nstep
+ *   history: HR_HISTORY_* ; segment: checkpoint interval (HR_HISTORY_CHECKPOINT) */
+int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, const long long* src_eq,
+               const double* src_val, int nrec, const long long* rec_eq, double* traces, int history,
+               int segment);
+
+/* OtD adjoint + gradient (Adjoint_OtD, Control_OtD, g <- -g).  Uses the
+ * history of the last hr_forward.  dis_meas[(nstep+1) x nrec] measured data.
+ * g_lambda, g_mu: nodal (nnode), zero outside the regular-domain node box.
+ * misfit (optional, may be NULL): Compute_Misfit of the last forward. */
+int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long long* rec_eq,
+                        const double* dis_meas, double* g_lambda, double* g_mu, double* misfit);
+
+/* Compute_Misfit of the traces of the last hr_forward against dis_meas. */
+int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_meas, double* misfit);
+
+/* Forward steps recomputed per adjoint (1.0 with a stored history). */
+double hr_recompute_ratio(const hr_ctx* ctx);
+
+/* Stage-kernel tiling (elements per z-chunk); 0 keeps the default. */
+int hr_set_tiling(hr_ctx* ctx, int tz);
+
+/* Stage-kernel timing with HIP events recorded around every stage-kernel
+ * launch on the context's stream (off by default; on = 1, reset = 2).
+ * hr_get_kernel_stats fills launches[5] and ms[5] per stage kind
+ * (0 = single-operator apply, 1..4 = RK4 stage). */
+int hr_set_profiling(hr_ctx* ctx, int mode);
+int hr_get_kernel_stats(hr_ctx* ctx, long long* launches, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIRENSI_H */
