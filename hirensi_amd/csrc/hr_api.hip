@@ -18,8 +18,9 @@
 using namespace hr;
 
 namespace {
-constexpr int TX = 4, TY = 4;
 constexpr int DEFAULT_TZ = 8;
+constexpr int DEFAULT_TY = 3;
+constexpr int DEFAULT_VERSION = 2;
 
 struct HipError {
   hipError_t e;
@@ -86,6 +87,10 @@ struct hr_ctx {
   size_t obs_cap = 0, sorted_cap = 0;
   double recompute = 1.0;
   hipStream_t stream = nullptr;
+  // stage-kernel version and tile-class lists (v2)
+  int version = DEFAULT_VERSION;
+  int* d_tl[3] = {nullptr, nullptr, nullptr};
+  int ntl[3] = {0, 0, 0};
   // stage-kernel timing
   int profiling = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -165,8 +170,8 @@ long long lat(const Geo& g, int i, int j, int k) {
 }
 
 int tile_of(const Geo& g, int i, int j, int k) {
-  const int tx = std::min(i / (2 * TX), g.tiles[0] - 1);
-  const int ty = std::min(j / (2 * TY), g.tiles[1] - 1);
+  const int tx = std::min(i / (2 * g.tx), g.tiles[0] - 1);
+  const int ty = std::min(j / (2 * g.ty), g.tiles[1] - 1);
   const int tz = std::min(k / (2 * g.tz), g.tiles[2] - 1);
   return tx + g.tiles[0] * (ty + g.tiles[1] * tz);
 }
@@ -253,7 +258,11 @@ int launch_stage(hr_ctx* ctx, StageArgs& A) {
     if (!e0 || !e1) return fail(ctx, 3, "hipEventCreate failed");
     HIPC(hipEventRecord(e0, ctx->stream));
   }
-  HIPC(launch_stage(A, ntiles(ctx->g), ctx->stream));
+  if (ctx->version == 1) {
+    HIPC(launch_stage(A, ntiles(ctx->g), ctx->stream));
+  } else {
+    for (int cls = 0; cls < 3; ++cls) HIPC(launch_stage2(A, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
+  }
   if (ctx->profiling) {
     HIPC(hipEventRecord(e1, ctx->stream));
     ctx->ev_open.push_back({A.stage, {e0, e1}});
@@ -270,6 +279,37 @@ StageArgs base_args(hr_ctx* ctx) {
   A.nd.minv_u = ctx->d_minv; A.nd.rwp = ctx->d_rwp; A.nd.wps = ctx->d_wps;
   A.sgn[0] = A.sgn[1] = A.sgn[2] = 1.0;
   return A;
+}
+
+// Tile grid and the three tile-class lists of the v2 stage kernel.  A tile's
+// window is its elements plus the low-side halo element in x, y and z.
+int setup_tiles(hr_ctx* ctx) {
+  Geo& g = ctx->g;
+  g.tiles[0] = (g.nex + g.tx - 1) / g.tx;
+  g.tiles[1] = (g.ney + g.ty - 1) / g.ty;
+  g.tiles[2] = (g.nez + g.tz - 1) / g.tz;
+  std::vector<int> lists[3];
+  const int nt = ntiles(g);
+  for (int b = 0; b < nt; ++b) {
+    const int tx = b % g.tiles[0], ty = (b / g.tiles[0]) % g.tiles[1], tz = b / (g.tiles[0] * g.tiles[1]);
+    bool has_rd = false, has_pml = false;
+    for (int ez = std::max(tz * g.tz - 1, 0); ez < std::min((tz + 1) * g.tz, g.nez); ++ez)
+      for (int ey = std::max(ty * g.ty - 1, 0); ey < std::min((ty + 1) * g.ty, g.ney); ++ey)
+        for (int ex = std::max(tx * g.tx - 1, 0); ex < std::min((tx + 1) * g.tx, g.nex); ++ex) {
+          const bool rd = ex >= g.rlo[0] && ex < g.rhi[0] && ey >= g.rlo[1] && ey < g.rhi[1] && ez >= g.rlo[2];
+          if (rd) has_rd = true; else has_pml = true;
+        }
+    lists[(has_rd && has_pml) ? 0 : (has_rd ? 1 : 2)].push_back(b);
+  }
+  for (int c = 0; c < 3; ++c) {
+    ctx->ntl[c] = (int)lists[c].size();
+    if (int rc = upload(ctx, &ctx->d_tl[c], lists[c])) return rc;
+  }
+  // per-tile source lists of an earlier forward no longer match the tiles
+  if (ctx->d_src_tile_ptr) { (void)hipFree(ctx->d_src_tile_ptr); ctx->d_src_tile_ptr = nullptr; }
+  ctx->f_nstep = 0;
+  ctx->f_history = HR_HISTORY_NONE;
+  return 0;
 }
 
 int check_ready(hr_ctx* ctx) {
@@ -399,10 +439,9 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
   g.rlo[1] = d->npml[2]; g.rhi[1] = g.ney - d->npml[3];
   g.rlo[2] = d->npml[4]; g.rhi[2] = g.nez;
   g.top_rule = d->top_stress_bc ? 1 : 0;
+  g.tx = 4;
+  g.ty = DEFAULT_TY;
   g.tz = DEFAULT_TZ;
-  g.tiles[0] = (g.nex + TX - 1) / TX;
-  g.tiles[1] = (g.ney + TY - 1) / TY;
-  g.tiles[2] = (g.nez + g.tz - 1) / g.tz;
   // affine element: DJ = diag(h/2); DETJ and DJI as in ELEMENT_K_3D
   const double j11 = d->h[0] * 0.5, j22 = d->h[1] * 0.5, j33 = d->h[2] * 0.5;
   const double detj = j11 * j22 * j33 + 0.0 * 0.0 * 0.0 + 0.0 * 0.0 * 0.0 - 0.0 * j22 * 0.0 - 0.0 * 0.0 * j11 - j33 * 0.0 * 0.0;
@@ -428,6 +467,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
           if (h_active(g, i, j, k, c)) ctx->eq_slot.push_back((long long)c * g.NN + lat(g, i, j, k));
   ctx->neq = (long long)ctx->eq_slot.size();
   HIPC(hipStreamCreate(&ctx->stream));
+  if (int rc = setup_tiles(ctx)) return rc;
   if (int rc = upload(ctx, &ctx->d_eq_slot, ctx->eq_slot)) return rc;
   // coordinates (origin: centred in x, y; top surface at z = 0)
   std::vector<double> xc(g.NX), yc(g.NY), zc(g.NZ);
@@ -465,6 +505,7 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   F(ctx->d_rec_tile_ptr); F(ctx->d_rec_node); F(ctx->d_rec_comp); F(ctx->d_rec_m); F(ctx->d_rec_perm);
   F(ctx->d_obs); F(ctx->d_gl); F(ctx->d_gm); F(ctx->d_part); F(ctx->d_scalar); F(ctx->d_tr_sorted);
   F(ctx->d_obs_sorted);
+  for (int c = 0; c < 3; ++c) F(ctx->d_tl[c]);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& p : ctx->ev_open) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -654,17 +695,19 @@ int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const doubl
   return 0;
 }
 
-int hr_set_tiling(hr_ctx* ctx, int tz) {
+int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   if (!ctx) return 1;
-  if (tz < 0) return fail(ctx, 1, "bad tiling");
+  if (ty < 0 || tz < 0 || version < 0 || version > 2) return fail(ctx, 1, "bad tiling");
   Geo& g = ctx->g;
+  const int v = version ? version : DEFAULT_VERSION;
+  const int t = ty ? ty : (v == 1 ? 4 : DEFAULT_TY);
+  if (v == 1 && t != 4) return fail(ctx, 1, "stage kernel v1 tiles are 4 x 4 elements");
+  if (v == 2 && t != 3 && t != 4) return fail(ctx, 1, "stage kernel v2 tiles are 4 x 3 or 4 x 4 elements");
+  ctx->version = v;
+  g.tx = 4;
+  g.ty = t;
   g.tz = tz ? tz : DEFAULT_TZ;
-  g.tiles[2] = (g.nez + g.tz - 1) / g.tz;
-  // per-tile source lists of an earlier forward no longer match the tiles
-  if (ctx->d_src_tile_ptr) { (void)hipFree(ctx->d_src_tile_ptr); ctx->d_src_tile_ptr = nullptr; }
-  ctx->f_nstep = 0;
-  ctx->f_history = HR_HISTORY_NONE;
-  return 0;
+  return setup_tiles(ctx);
 }
 
 int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, const long long* src_eq,

@@ -15,6 +15,7 @@ struct Geo {
   int rlo[3], rhi[3];              // regular-domain element box [lo, hi)
   int top_rule;                    // 1: original top PML nodes lose s_zz, s_xz, s_yz
   int tiles[3];                    // stage-kernel tiles per axis
+  int tx, ty;                      // elements per tile in x and y
   int tz;                          // elements per z-chunk of a tile
   double dxi[3];                   // dxi/dx = DJI(k,k) of the affine element
   double wq[27];                   // GLL weight * det J at local point a + 3b + 9c
@@ -62,6 +63,8 @@ struct StageArgs {
 namespace hr {
 // Host launchers (defined with the kernels).
 hipError_t launch_stage(const StageArgs& A, int nblocks, hipStream_t s);
+// v2: tile class 0 = mixed, 1 = regular domain only, 2 = PML only; list of tile ids
+hipError_t launch_stage2(const StageArgs& A, int tile_class, const int* tiles, int ntiles, hipStream_t s);
 hipError_t launch_gather_eq(long long neq, const long long* slot, const double* x, const double* r, double* out,
                             hipStream_t s);
 hipError_t launch_scatter_eq(long long neq, const long long* slot, const double* raw, const double* y, double sc,

@@ -139,10 +139,10 @@ module hirensi
       type(c_ptr), value :: ctx
     end function hr_recompute_ratio
 
-    integer(c_int) function hr_set_tiling(ctx, tz) bind(C, name="hr_set_tiling")
+    integer(c_int) function hr_set_tiling(ctx, ty, tz, version) bind(C, name="hr_set_tiling")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
-      integer(c_int), value :: tz
+      integer(c_int), value :: ty, tz, version
     end function hr_set_tiling
 
     integer(c_int) function hr_set_profiling(ctx, mode) bind(C, name="hr_set_profiling")

@@ -70,7 +70,7 @@ def load():
     L.hr_misfit.argtypes = [_vp, C.c_int, C.c_double, C.c_int, _dp, _dp]
     L.hr_recompute_ratio.argtypes = [_vp]
     L.hr_recompute_ratio.restype = C.c_double
-    L.hr_set_tiling.argtypes = [_vp, C.c_int]
+    L.hr_set_tiling.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.hr_set_profiling.argtypes = [_vp, C.c_int]
     L.hr_get_kernel_stats.argtypes = [_vp, _dp, _dp]
     for name in EXPORTS:

@@ -62,7 +62,7 @@ class Context:
     passed to the methods must be device memory (e.g. torch tensors on cuda).
     """
 
-    def __init__(self, mesh, model=None, device_ptrs=False, tz=0):
+    def __init__(self, mesh, model=None, device_ptrs=False, tz=0, ty=0, version=0):
         self.L = native.load()
         self.mesh = mesh
         self.device_ptrs = bool(device_ptrs)
@@ -78,8 +78,8 @@ class Context:
         rc = self.L.hr_ctx_create(C.byref(d), C.byref(h))
         self.h = h
         self._check(rc)
-        if tz:
-            self._check(self.L.hr_set_tiling(self.h, int(tz)))
+        if tz or ty or version:
+            self._check(self.L.hr_set_tiling(self.h, int(ty), int(tz), int(version)))
         neq, nn, ns = C.c_longlong(), C.c_longlong(), C.c_longlong()
         self._check(self.L.hr_get_sizes(self.h, C.byref(neq), C.byref(nn), C.byref(ns)))
         self.neq, self.nnode, self.nstore = neq.value, nn.value, ns.value

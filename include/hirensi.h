@@ -118,8 +118,11 @@ int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_mea
 /* Forward steps recomputed per adjoint (1.0 with a stored history). */
 double hr_recompute_ratio(const hr_ctx* ctx);
 
-/* Stage-kernel tiling (elements per z-chunk); 0 keeps the default. */
-int hr_set_tiling(hr_ctx* ctx, int tz);
+/* Stage-kernel configuration: tile of 4 x ty elements in x-y, tz elements
+ * per z-chunk, kernel version (1: single-class tiles of 4 x 4; 2: tile
+ * classes, ty = 3 or 4).  0 keeps a default.  Invalidates the history of the
+ * last hr_forward. */
+int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version);
 
 /* Stage-kernel timing with HIP events recorded around every stage-kernel
  * launch on the context's stream (off by default; on = 1, reset = 2).

@@ -15,6 +15,8 @@ from tests.helpers import small_case, rel_l2
 pytestmark = pytest.mark.gpu
 
 OPS = {S.HR_K: 0, S.HR_C: 1, S.HR_G: 2}
+# stage-kernel configurations: (version, tile y-extent)
+KCFG = [(1, 4), (2, 3), (2, 4)]
 
 
 def _abs_bound(A, x, y):
@@ -24,13 +26,14 @@ def _abs_bound(A, x, y):
 @pytest.mark.parametrize("shape,npml,top", [((8, 8, 6), 2, "reference"), ((7, 9, 5), (1, 2, 2, 1, 2), "none"),
                                             ((13, 10, 11), 3, "reference")])
 @pytest.mark.parametrize("transpose", [0, 1])
-def test_apply_matches_oracle_csr(shape, npml, top, transpose):
+@pytest.mark.parametrize("kcfg", KCFG)
+def test_apply_matches_oracle_csr(shape, npml, top, transpose, kcfg):
     npml = (npml,) * 5 if isinstance(npml, int) else npml
     mesh = BoxMesh(*shape, npml=npml, top_stress_bc=top)
     from tests.helpers import random_model
     model = random_model(mesh, seed=3)
-    print(f"apply case shape={shape} npml={npml} top={top} transpose={transpose}", flush=True)
-    ctx = S.Context(mesh, model)
+    print(f"apply case shape={shape} npml={npml} top={top} transpose={transpose} kernel={kcfg}", flush=True)
+    ctx = S.Context(mesh, model, version=kcfg[0], ty=kcfg[1])
     sysm = O.System(mesh, model)
     rng = np.random.default_rng(11)
     for which, w in OPS.items():
@@ -96,8 +99,8 @@ def test_dot_product_identity():
         assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
 
 
-def _forward_both(mesh, model, rec_eq, src_eq, src_val, nstep, dt, fr=15.0, tz=0):
-    ctx = S.Context(mesh, model, tz=tz)
+def _forward_both(mesh, model, rec_eq, src_eq, src_val, nstep, dt, fr=15.0, tz=0, kcfg=(0, 0)):
+    ctx = S.Context(mesh, model, tz=tz, version=kcfg[0], ty=kcfg[1])
     ftab = S.time_table(nstep, dt, 2, fr)
     tr = S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, ftab, src_eq, src_val, rec_eq)
     sysm = O.System(mesh, model)
@@ -111,11 +114,13 @@ def _forward_both(mesh, model, rec_eq, src_eq, src_val, nstep, dt, fr=15.0, tz=0
 
 
 @pytest.mark.parametrize("tz", [0, 1, 3])
-def test_forward_traces_match_oracle(tz):
+@pytest.mark.parametrize("kcfg", KCFG)
+def test_forward_traces_match_oracle(tz, kcfg):
     mesh, model, rec_eq, src_eq, src_val = small_case()
     nstep, dt = 300, 1.0e-4
-    print(f"forward case tz={tz} neq={mesh.neq} nstep={nstep}", flush=True)
-    ctx, sysm, tr, U, rows = _forward_both(mesh, model, rec_eq, src_eq, src_val, nstep, dt, fr=60.0, tz=tz)
+    print(f"forward case tz={tz} kernel={kcfg} neq={mesh.neq} nstep={nstep}", flush=True)
+    ctx, sysm, tr, U, rows = _forward_both(mesh, model, rec_eq, src_eq, src_val, nstep, dt, fr=60.0, tz=tz,
+                                           kcfg=kcfg)
     ref = U[:, rows]
     assert np.abs(ref).max() > 0
     assert rel_l2(tr, ref) < 1e-10
@@ -146,13 +151,14 @@ def _observed(mesh, rec_eq, src_eq, src_val, nstep, dt, fr, seed):
     return U[:, rows].copy()
 
 
-@pytest.mark.parametrize("history", [S.HR_HISTORY_STORE, S.HR_HISTORY_CHECKPOINT])
-def test_adjoint_gradient_matches_oracle(history):
+@pytest.mark.parametrize("history,kcfg", [(S.HR_HISTORY_STORE, (1, 4)), (S.HR_HISTORY_STORE, (2, 3)),
+                                          (S.HR_HISTORY_CHECKPOINT, (2, 3)), (S.HR_HISTORY_STORE, (2, 4))])
+def test_adjoint_gradient_matches_oracle(history, kcfg):
     mesh, model, rec_eq, src_eq, src_val = small_case(seed=1)
     nstep, dt, fr = 240, 1.0e-4, 60.0
-    print(f"adjoint case history={history} neq={mesh.neq}", flush=True)
+    print(f"adjoint case history={history} kernel={kcfg} neq={mesh.neq}", flush=True)
     dmeas = _observed(mesh, rec_eq, src_eq, src_val, nstep, dt, fr, seed=9)
-    ctx = S.Context(mesh, model)
+    ctx = S.Context(mesh, model, version=kcfg[0], ty=kcfg[1])
     ftab = S.time_table(nstep, dt, 2, fr)
     S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, ftab, src_eq, src_val, rec_eq, history=history, segment=37)
     gl, gm, mis = S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dmeas)
