@@ -31,6 +31,8 @@ def parse():
     p.add_argument("--history", choices=["store", "checkpoint"], default="store")
     p.add_argument("--segment", type=int, default=64)
     p.add_argument("--tz", type=int, default=0)
+    p.add_argument("--ty", type=int, default=0)
+    p.add_argument("--version", type=int, default=0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=12, help="oracle RK4 steps timed (each direction)")
     p.add_argument("--copy-calib", action="store_true", help="also run a plain 1 GiB device copy (PMC scale)")
@@ -101,7 +103,7 @@ def main():
     def T(x, dtype=torch.float64):
         return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=dev)
 
-    ctx = S.Context(mesh, device_ptrs=True, tz=a.tz)
+    ctx = S.Context(mesh, device_ptrs=True, tz=a.tz, ty=a.ty, version=a.version)
     d_ftab, d_src, d_val, d_rec = T(ftab), T(src_eq, torch.int64), T(src_val), T(rec_eq, torch.int64)
     d_obs = torch.zeros((nstep + 1, nrec), dtype=torch.float64, device=dev)
     d_tr = torch.zeros_like(d_obs)
@@ -170,7 +172,8 @@ def main():
                    "gradient_wall_s": wall, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stage_kernel<4,4>", "launches": launches, "avg_launch_us": avg_us,
+                     "kernel": "stage kernel (all tile classes of one RK4 stage)", "launches": launches,
+                     "avg_launch_us": avg_us,
                      "alg_bytes_per_launch": alg_bytes / launches,
                      "stage_kernel_share_of_wall": kern_ms * 1e-3 / (wall * a.steps)},
     }

@@ -1001,46 +1001,54 @@ __global__ void gradient_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int b
   const long long nbox = (long long)bnx * bny * bnz;
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nbox; r += (long long)gridDim.x * blockDim.x) {
     const int bi = (int)(r % bnx), bj = (int)((r / bnx) % bny), bk = (int)(r / ((long long)bnx * bny));
-    const int i = bi0 + bi, j = bj0 + bj, k = bk0 + bk;
-    // element sides per axis
-    int es[3][2], ls[3][2], ns[3];
-    const int pos[3] = {i, j, k};
-    // only sides whose element lies in the regular-domain range of that axis:
-    // their stencils stay inside the regular-domain node box
-    for (int a = 0; a < 3; ++a) {
-      ns[a] = 0;
-      const int q = pos[a];
-      if (q & 1) { es[a][0] = (q - 1) >> 1; ls[a][0] = 1; ns[a] = 1; }
-      else {
-        const int el = (q >> 1) - 1, er = q >> 1;
-        if (el >= g.rlo[a] && el < g.rhi[a]) { es[a][ns[a]] = el; ls[a][ns[a]] = 2; ++ns[a]; }
-        if (er >= g.rlo[a] && er < g.rhi[a]) { es[a][ns[a]] = er; ls[a][ns[a]] = 0; ++ns[a]; }
-      }
-    }
-    // derivatives per axis and side: du[a][s][c], dp[a][s][c]
-    double du[3][2][3], dp[3][2][3];
+    const int pos[3] = {bi0 + bi, bj0 + bj, bk0 + bk};
+    const long long n = (long long)pos[0] + (long long)g.NX * ((long long)pos[1] + (long long)g.NY * pos[2]);
     const long long stride[3] = {1, bnx, (long long)bnx * bny};
     const long long lstride[3] = {1, g.NX, (long long)g.NX * g.NY};
-    const long long n = (long long)i + (long long)g.NX * ((long long)j + (long long)g.NY * k);
-    for (int a = 0; a < 3; ++a)
-      for (int s = 0; s < ns[a]; ++s) {
-        const int l = ls[a][s];
+    // per axis two element sides: s = 0 the element below (local 2; local 1 for an
+    // odd node), s = 1 the element above (local 0).  Only sides whose element is
+    // in the regular-domain range of the axis are used: their stencils stay in
+    // the regular-domain node box.
+    bool has[3][2];
+    int el[3][2], lo[3][2];
+    double du[3][2][3], dp[3][2][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int q = pos[a];
+      const bool odd = q & 1;
+      el[a][0] = odd ? (q - 1) >> 1 : (q >> 1) - 1;
+      lo[a][0] = odd ? 1 : 2;
+      el[a][1] = q >> 1;
+      lo[a][1] = 0;
+      has[a][0] = el[a][0] >= g.rlo[a] && el[a][0] < g.rhi[a];
+      has[a][1] = !odd && el[a][1] >= g.rlo[a] && el[a][1] < g.rhi[a];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int l = lo[a][s];
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
           double su = 0.0, sp = 0.0;
-          for (int m = 0; m < 3; ++m) {
-            const long long off = (long long)(m - l);
-            su += DM(l, m) * uh[c * nbox + r + off * stride[a]];
-            sp += DM(l, m) * p2[c * g.NN + n + off * lstride[a]];
+          if (has[a][s]) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+              const long long off = (long long)(m - l);
+              su += DM(l, m) * uh[c * nbox + r + off * stride[a]];
+              sp += DM(l, m) * p2[c * g.NN + n + off * lstride[a]];
+            }
           }
           du[a][s][c] = g.dxi[a] * su;
           dp[a][s][c] = g.dxi[a] * sp;
         }
       }
+    }
     double sl = 0.0, sm = 0.0;
-    for (int sx = 0; sx < ns[0]; ++sx)
-      for (int sy = 0; sy < ns[1]; ++sy)
-        for (int sz = 0; sz < ns[2]; ++sz) {
-          if (!rd_elem(g, es[0][sx], es[1][sy], es[2][sz])) continue;
+#pragma unroll
+    for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+      for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+        for (int sz = 0; sz < 2; ++sz) {
+          if (!(has[0][sx] && has[1][sy] && has[2][sz])) continue;
           const double u1x = du[0][sx][0], u2x = du[0][sx][1], u3x = du[0][sx][2];
           const double u1y = du[1][sy][0], u2y = du[1][sy][1], u3y = du[1][sy][2];
           const double u1z = du[2][sz][0], u2z = du[2][sz][1], u3z = du[2][sz][2];
@@ -1050,7 +1058,7 @@ __global__ void gradient_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int b
           const double sL = (u1x + u2y + u3z) * (p1x + p2y + p3z);
           const double sM = 2.0 * (u1x * p1x + u2y * p2y + u3z * p3z) + (p1y + p2x) * (u1y + u2x) +
                             (p1z + p3x) * (u1z + u3x) + (p3y + p2z) * (u3y + u2z);
-          const double W = g.wq[ls[0][sx] + 3 * ls[1][sy] + 9 * ls[2][sz]];
+          const double W = g.wq[lo[0][sx] + 3 * lo[1][sy] + 9 * lo[2][sz]];
           sl += sL * W;
           sm += sM * W;
         }
