@@ -26,6 +26,13 @@ struct HipError {
   hipError_t e;
 };
 
+// per-launch source parameters of the v3 stage kernel
+struct Src3Spec {
+  const Src3* dev = nullptr;
+  int mode = 0, ra = 0, rb = -1;
+  double ca = 0.0, cb = 0.0;
+};
+
 struct Shell {
   double s = 1.0;
   double* l = nullptr;   // device, neq (nullptr = identity)
@@ -47,6 +54,10 @@ struct hr_ctx {
   // node data (device)
   double *d_xc = nullptr, *d_yc = nullptr, *d_zc = nullptr;
   double *d_lam = nullptr, *d_mu = nullptr, *d_minv = nullptr, *d_rwp = nullptr, *d_wps = nullptr;
+  double* d_node = nullptr;                 // v3 node table (NT_COUNT x NN)
+  double* d_tab = nullptr;                  // v3 small table (weights, per-axis stretch)
+  Src3* d_src3_fwd = nullptr;               // v3 source lists (device structs)
+  Src3* d_src3_adj = nullptr;
   std::vector<double> h_minv_u, h_wps, h_a;  // host copies for M^-1 of sources, diag M
   bool have_material = false, have_pml = false;
   // state (device, 9*NN each)
@@ -249,7 +260,31 @@ int drain_events(hr_ctx* ctx) {
   return 0;
 }
 
-int launch_stage(hr_ctx* ctx, StageArgs& A) {
+Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
+  Stage3Args B;
+  std::memset(&B, 0, sizeof(B));
+  const Geo& g = ctx->g;
+  B.g.NX = g.NX; B.g.NY = g.NY; B.g.NZ = g.NZ; B.g.NN = (int)g.NN;
+  B.g.nex = g.nex; B.g.ney = g.ney; B.g.nez = g.nez;
+  for (int a = 0; a < 3; ++a) B.g.rlo[a] = g.rlo[a];
+  B.g.rhi[0] = g.rhi[0]; B.g.rhi[1] = g.rhi[1];
+  B.g.top_rule = g.top_rule;
+  B.g.tiles0 = g.tiles[0]; B.g.tiles1 = g.tiles[1]; B.g.tz = g.tz;
+  for (int a = 0; a < 3; ++a) B.g.dxi[a] = g.dxi[a];
+  for (int q = 0; q < 6; ++q) B.in[q] = A.in[q];
+  B.out = A.out;
+  for (int q = 0; q < 3; ++q) B.xnew[q] = A.xnew[q];
+  B.node = ctx->d_node;
+  B.tab = ctx->d_tab;
+  B.set_mask = A.set_mask;
+  B.hh = A.hh; B.h = A.h; B.h6 = A.h6;
+  if (sp && sp->mode) {
+    B.src = sp->dev; B.src_mode = sp->mode; B.ra = sp->ra; B.rb = sp->rb; B.ca = sp->ca; B.cb = sp->cb;
+  }
+  return B;
+}
+
+int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ctx->profiling) {
     if (ctx->ev_open.size() >= 4096) if (int rc = drain_events(ctx)) return rc;
@@ -260,6 +295,19 @@ int launch_stage(hr_ctx* ctx, StageArgs& A) {
   }
   if (ctx->version == 1) {
     HIPC(launch_stage(A, ntiles(ctx->g), ctx->stream));
+  } else if (ctx->version == 3) {
+    // 32-bit byte offsets: the node table (6 NN) and the 9 components must stay below 4 GiB
+    if ((unsigned long long)ctx->g.NN * 8ull * 9ull >= (1ull << 32))
+      return fail(ctx, 3, "lattice of %lld nodes is too large for the v3 stage kernel", ctx->g.NN);
+    // the v3 kernel fixes the adjoint signs by the transpose flag of stages 1-4
+    const bool adj_signs = A.sgn[0] == -1.0 && A.sgn[1] == 1.0 && A.sgn[2] == -1.0;
+    const bool unit_signs = A.sgn[0] == 1.0 && A.sgn[1] == 1.0 && A.sgn[2] == 1.0;
+    if (!((A.stage == 0 && unit_signs) || (A.stage > 0 && (A.transpose ? adj_signs : unit_signs))))
+      return fail(ctx, 3, "stage sign pattern not supported by the v3 kernel");
+    if (A.nsrc_tiles && !(sp && sp->mode)) return fail(ctx, 3, "v3 stage launch without its source lists");
+    const Stage3Args B = args3(ctx, A, sp);
+    for (int cls = 0; cls < 3; ++cls)
+      HIPC(launch_stage3(B, A.stage, A.transpose, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
   } else {
     for (int cls = 0; cls < 3; ++cls) HIPC(launch_stage2(A, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
   }
@@ -330,7 +378,8 @@ int zero_state(hr_ctx* ctx) {
 }
 
 // One RK4 step on (x) -> (y), then swap.  dir = +1 forward, -1 adjoint.
-int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3], const Source* src4) {
+int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3], const Source* src4,
+             const Src3Spec* sp4) {
   const double hh = dir * (0.5 * dt), h = dir * dt, h6 = dir * (dt / 6.0);
   for (int s = 1; s <= 4; ++s) {
     StageArgs A = base_args(ctx);
@@ -344,7 +393,7 @@ int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3]
     if (s < 4) A.out = ctx->gs[s - 1];
     else { A.xnew[0] = ctx->y[0]; A.xnew[1] = ctx->y[1]; A.xnew[2] = ctx->y[2]; }
     if (src4 && src4[s - 1].tile_ptr) { A.src = src4[s - 1]; A.nsrc_tiles = 1; }
-    if (int rc = launch_stage(ctx, A)) return rc;
+    if (int rc = launch_stage(ctx, A, sp4 ? &sp4[s - 1] : nullptr)) return rc;
   }
   std::swap(ctx->x, ctx->y);
   return 0;
@@ -363,9 +412,10 @@ int store_slice(hr_ctx* ctx, double* dst) {
 }
 
 // forward sources of step n: a = B values, ca = f
-void forward_sources(hr_ctx* ctx, int n, Source s4[4]) {
+void forward_sources(hr_ctx* ctx, int n, Source s4[4], Src3Spec sp4[4]) {
   for (int s = 0; s < 4; ++s) {
     std::memset(&s4[s], 0, sizeof(Source));
+    sp4[s] = Src3Spec();
     if (!ctx->d_src_tile_ptr) continue;
     s4[s].tile_ptr = ctx->d_src_tile_ptr;
     s4[s].node = ctx->d_src_node;
@@ -374,6 +424,9 @@ void forward_sources(hr_ctx* ctx, int n, Source s4[4]) {
     s4[s].a = ctx->d_src_val;
     const double* f = &ctx->ftab[3 * (size_t)n];
     s4[s].ca = (s == 0) ? f[0] : (s == 3 ? f[2] : f[1]);
+    sp4[s].dev = ctx->d_src3_fwd;
+    sp4[s].mode = 1;
+    sp4[s].ca = s4[s].ca;
   }
 }
 
@@ -383,8 +436,9 @@ int run_forward_steps(hr_ctx* ctx, int n0, int n1, double* seg_hist) {
   for (int n = n0; n < n1; ++n) {
     if (seg_hist) if (int rc = store_slice(ctx, seg_hist + (size_t)(n - n0) * 3 * ctx->nbox)) return rc;
     Source s4[4];
-    forward_sources(ctx, n, s4);
-    if (int rc = rk4_step(ctx, ctx->f_dt, +1, 0, sgn, s4)) return rc;
+    Src3Spec sp4[4];
+    forward_sources(ctx, n, s4, sp4);
+    if (int rc = rk4_step(ctx, ctx->f_dt, +1, 0, sgn, s4, sp4)) return rc;
   }
   if (seg_hist) if (int rc = store_slice(ctx, seg_hist + (size_t)(n1 - n0) * 3 * ctx->nbox)) return rc;
   return 0;
@@ -498,6 +552,7 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(ctx->d_eq_slot); F(ctx->d_xc); F(ctx->d_yc); F(ctx->d_zc); F(ctx->d_lam); F(ctx->d_mu); F(ctx->d_minv);
   F(ctx->d_rwp); F(ctx->d_wps);
+  F(ctx->d_node); F(ctx->d_tab); F(ctx->d_src3_fwd); F(ctx->d_src3_adj);
   for (int a = 0; a < 3; ++a) { F(ctx->x[a]); F(ctx->y[a]); F(ctx->gs[a]); F(ctx->shell[a].l); F(ctx->shell[a].r); }
   F(ctx->tmp); F(ctx->tmp2); F(ctx->d_traces); F(ctx->d_rec_slot); F(ctx->d_hist); F(ctx->d_seg);
   for (double* p : ctx->ckpt) F(p);
@@ -545,6 +600,8 @@ int hr_set_pml(hr_ctx* ctx, const double* p) {
   if (!(p[22] >= 0.0)) return fail(ctx, 1, "top PML zone (face 5) would contain nodes");
   std::memcpy(ctx->g.pml, p, sizeof(double) * 24);
   ctx->have_pml = true;
+  // the mass inverses and stretch tables depend on PML_PARAM: hr_set_material must follow
+  ctx->have_material = false;
   return 0;
 }
 
@@ -599,6 +656,28 @@ int hr_set_material(hr_ctx* ctx, const double* lam, const double* mu, const doub
         ctx->h_minv_u[n] = M > 0.0 ? 1.0 / M : 0.0;
       }
   ctx->h_wps = wps;
+  // v3 tables: node table [lam mu minv_u minv_s rwp wps] and [wq | per-axis stretch a, b]
+  {
+    std::vector<double> nt((size_t)NT_COUNT * NN, 0.0);
+    for (size_t n = 0; n < NN; ++n) {
+      nt[NT_LAM * NN + n] = hl[n];
+      nt[NT_MU * NN + n] = hm[n];
+      nt[NT_MINV_U * NN + n] = ctx->h_minv_u[n];
+      nt[NT_MINV_S * NN + n] = wps[n] > 0.0 ? 1.0 / (ctx->h_a[n] * wps[n]) : 0.0;
+      nt[NT_RWP * NN + n] = rwp[n];
+      nt[NT_WPS * NN + n] = wps[n];
+    }
+    if (int rc = upload(ctx, &ctx->d_node, nt)) return rc;
+    std::vector<double> tb(TAB_STRETCH + 2 * (size_t)(g.NX + g.NY + g.NZ));
+    for (int q = 0; q < 27; ++q) tb[q] = g.wq[q];
+    double* ax = tb.data() + TAB_STRETCH;
+    double* ay = ax + 2 * g.NX;
+    double* az = ay + 2 * g.NY;
+    for (int i = 0; i < g.NX; ++i) { double s[6]; h_stretch(g.pml, xc[i], yc[0], zc[g.NZ - 1], s); ax[i] = s[0]; ax[g.NX + i] = s[3]; }
+    for (int j = 0; j < g.NY; ++j) { double s[6]; h_stretch(g.pml, xc[0], yc[j], zc[g.NZ - 1], s); ay[j] = s[1]; ay[g.NY + j] = s[4]; }
+    for (int k = 0; k < g.NZ; ++k) { double s[6]; h_stretch(g.pml, xc[0], yc[0], zc[k], s); az[k] = s[2]; az[g.NZ + k] = s[5]; }
+    if (int rc = upload(ctx, &ctx->d_tab, tb)) return rc;
+  }
   if (int rc = upload(ctx, &ctx->d_lam, hl)) return rc;
   if (int rc = upload(ctx, &ctx->d_mu, hm)) return rc;
   if (int rc = upload(ctx, &ctx->d_minv, ctx->h_minv_u)) return rc;
@@ -697,12 +776,13 @@ int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const doubl
 
 int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   if (!ctx) return 1;
-  if (ty < 0 || tz < 0 || version < 0 || version > 2) return fail(ctx, 1, "bad tiling");
+  if (ty < 0 || tz < 0 || version < 0 || version > 3) return fail(ctx, 1, "bad tiling");
   Geo& g = ctx->g;
   const int v = version ? version : DEFAULT_VERSION;
   const int t = ty ? ty : (v == 1 ? 4 : DEFAULT_TY);
   if (v == 1 && t != 4) return fail(ctx, 1, "stage kernel v1 tiles are 4 x 4 elements");
   if (v == 2 && t != 3 && t != 4) return fail(ctx, 1, "stage kernel v2 tiles are 4 x 3 or 4 x 4 elements");
+  if (v == 3 && t != 3) return fail(ctx, 1, "stage kernel v3 tiles are 4 x 3 elements");
   ctx->version = v;
   g.tx = 4;
   g.ty = t;
@@ -756,6 +836,11 @@ int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, 
     if (int rc = upload(ctx, &ctx->d_src_comp, sb.comp)) return rc;
     if (int rc = upload(ctx, &ctx->d_src_m, sb.m)) return rc;
     if (int rc = upload(ctx, &ctx->d_src_val, v)) return rc;
+    Src3 h3;
+    std::memset(&h3, 0, sizeof(h3));
+    h3.tile_ptr = ctx->d_src_tile_ptr; h3.node = ctx->d_src_node; h3.comp = ctx->d_src_comp;
+    h3.m = ctx->d_src_m; h3.val = ctx->d_src_val;
+    if (int rc = upload(ctx, &ctx->d_src3_fwd, std::vector<Src3>{h3})) return rc;
   } else if (ctx->d_src_tile_ptr) {
     (void)hipFree(ctx->d_src_tile_ptr);
     ctx->d_src_tile_ptr = nullptr;
@@ -795,8 +880,9 @@ int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, 
         HIPC(hipMemcpyAsync(cp + a * nf, ctx->x[a], nf * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     }
     Source s4[4];
-    forward_sources(ctx, n, s4);
-    if (int rc = rk4_step(ctx, dt, +1, 0, sgn, s4)) return rc;
+    Src3Spec sp4[4];
+    forward_sources(ctx, n, s4, sp4);
+    if (int rc = rk4_step(ctx, dt, +1, 0, sgn, s4, sp4)) return rc;
   }
   if (int rc = record_step(ctx, nstep)) return rc;
   if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist + (size_t)nstep * slice)) return rc;
@@ -867,6 +953,13 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     if (int rc = dalloc(ctx, &ctx->d_obs_sorted, nobs)) return rc;
     ctx->sorted_cap = nobs;
   }
+  {
+    Src3 h3;
+    std::memset(&h3, 0, sizeof(h3));
+    h3.tile_ptr = ctx->d_rec_tile_ptr; h3.node = ctx->d_rec_node; h3.comp = ctx->d_rec_comp;
+    h3.m = ctx->d_rec_m; h3.tr = ctx->d_tr_sorted; h3.obs = ctx->d_obs_sorted; h3.nrec = nrec;
+    if (int rc = upload(ctx, &ctx->d_src3_adj, std::vector<Src3>{h3})) return rc;
+  }
   if (nrec) {
     HIPC(launch_permute_cols(nstep + 1, nrec, ctx->d_rec_perm, ctx->d_traces, ctx->d_tr_sorted, ctx->stream));
     HIPC(launch_permute_cols(nstep + 1, nrec, ctx->d_rec_perm, ctx->d_obs, ctx->d_obs_sorted, ctx->stream));
@@ -933,9 +1026,16 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     }
     // misfit sources at t_n (stage 1), t_{n-1} (stage 4), mean (stages 2, 3)
     Source s4[4];
+    Src3Spec sp4[4];
     for (int s = 0; s < 4; ++s) {
       std::memset(&s4[s], 0, sizeof(Source));
+      sp4[s] = Src3Spec();
       if (!nrec) continue;
+      sp4[s].dev = ctx->d_src3_adj;
+      sp4[s].mode = 2;
+      if (s == 0) { sp4[s].ra = n; sp4[s].ca = 1.0; }
+      else if (s == 3) { sp4[s].ra = n - 1; sp4[s].ca = 1.0; }
+      else { sp4[s].ra = n; sp4[s].ca = 0.5; sp4[s].rb = n - 1; sp4[s].cb = 0.5; }
       s4[s].tile_ptr = ctx->d_rec_tile_ptr;
       s4[s].node = ctx->d_rec_node;
       s4[s].comp = ctx->d_rec_comp;
@@ -952,7 +1052,7 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     double* fx[3] = {ctx->x[0], ctx->x[1], ctx->x[2]};
     double* fy[3] = {ctx->y[0], ctx->y[1], ctx->y[2]};
     for (int a = 0; a < 3; ++a) { ctx->x[a] = adj[a]; ctx->y[a] = adjy[a]; }
-    if (int rc = rk4_step(ctx, dt, -1, 1, sgn, s4)) { status = rc; }
+    if (int rc = rk4_step(ctx, dt, -1, 1, sgn, s4, sp4)) { status = rc; }
     for (int a = 0; a < 3; ++a) { adj[a] = ctx->x[a]; adjy[a] = ctx->y[a]; }
     if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
       for (int a = 0; a < 3; ++a) { ctx->x[a] = fx[a]; ctx->y[a] = fy[a]; }

@@ -81,3 +81,52 @@ hipError_t launch_permute_cols(long long nrows, int ncols, const long long* perm
 hipError_t launch_misfit(int nstep, int nrec, const double* tr, const double* ob, double dt, double* part,
                          double* out, hipStream_t s);
 }  // namespace hr
+
+namespace hr {
+// ----------------------------------------------------------- stage kernel v3
+// Compact launch arguments: per-node and per-axis coefficients live in device
+// tables, source lists behind a device pointer.
+struct Geo3 {
+  int NX, NY, NZ, NN;              // lattice (9 * NN < 2^31)
+  int nex, ney, nez;
+  int rlo[3], rhi[2];              // regular-domain element box; it extends to the top in z
+  int top_rule;
+  int tiles0, tiles1, tz;
+  double dxi[3];
+};
+
+// Node table: [lam | mu | minv_u | minv_s | rwp | wps], NN entries each.
+constexpr int NT_LAM = 0, NT_MU = 1, NT_MINV_U = 2, NT_MINV_S = 3, NT_RWP = 4, NT_WPS = 5, NT_COUNT = 6;
+// Small table: [wq (27) | ax bx (NX each) | ay by (NY each) | az bz (NZ each)].
+constexpr int TAB_STRETCH = 27;
+
+struct Src3 {                      // device-resident source entries of one solve, grouped by tile
+  const int* tile_ptr;             // (ntiles + 1)
+  const long long* node;           // lattice node
+  const int* comp;                 // component 0..2
+  const double* m;                 // M^-1 of that DOF
+  const double* val;               // forward: B entries
+  const double* tr;                // adjoint: traces, columns in entry order, rows of nrec
+  const double* obs;               // adjoint: observations, same layout
+  int nrec;
+};
+
+struct Stage3Args {
+  Geo3 g;
+  const double* in[6];             // stage: x1 x2 x3 g1 g2 g3; apply: in[0] = x
+  double* out;                     // stage 1-3: g_s; apply: raw result
+  double* xnew[3];                 // stage 4: updated x1, x2, x3
+  const double* node;              // node table
+  const double* tab;               // small table
+  const Src3* src;                 // device pointer, used when src_mode != 0
+  int set_mask;                    // apply: bit0 G, bit1 K, bit2 C
+  int src_mode;                    // 0 none, 1 forward: ca * B * m, 2 adjoint misfit
+  int ra, rb;                      // adjoint source rows (rb < 0: none)
+  double ca, cb;
+  double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed)
+};
+
+// stage 0 = apply; transposed stages 1..4 carry the adjoint signs (-G, +K, -C)
+hipError_t launch_stage3(const Stage3Args& A, int stage, int transpose, int tile_class, const int* tiles, int ntiles,
+                         hipStream_t s);
+}  // namespace hr

@@ -1,0 +1,593 @@
+// Stage kernel v3: one RK4 stage (or one operator apply) of the explicit
+// elastic system with the hybrid displacement-stress PML, matrix-free on the
+// structured lattice of 27-node GLL hexahedra.
+//
+// The operator is the one the reference assembles into AK, AC, AG
+// (Assemble_PETSc_3D.F90:6-436, element blocks Element_Matrices_3D.F90:7-882)
+// and the stage is the VecWAXPY/MatMultAdd sequence of
+// Solver_PETSc_Explicit.F90:1001-1046 (forward) and :1319-1364 (adjoint,
+// transposed with the signs -G, +K, -C).
+//
+// Work decomposition (same tiles as the v2 kernel): a workgroup owns TX x TY
+// elements in x-y and a chunk of tz element layers, and streams up the chunk
+// one element layer (slab) at a time.  A slab's lattice window (owned nodes
+// plus the low-side halo element) sits in a three-plane LDS ring.
+//
+// Differences from v2, all aimed at instruction count:
+//  * the stage, the transpose and the tile class are template parameters;
+//  * PML stretch factors come from per-axis tables (a_x, b_x depend on x
+//    only, ...), the stress-row mass inverse from a node table: no division;
+//  * the PML strain rows use F-weighted inputs w_d = sum_X F[X][d] v_X, which
+//    is exact algebra because F[X][d] does not vary along the d-stencil
+//    (F[X][x] depends on y and z only, ...): 9 derivatives per element side
+//    instead of 27;
+//  * each owner thread loads its own node, keeps the diagonal terms (and for
+//    stage 4 x3 and the partial g-sum) in registers, and writes the x1, x2
+//    updates of stage 4 at load time; helpers load the low-side halo;
+//  * 32-bit node offsets on uniform per-component bases; small constant
+//    tables (GLL derivative matrix, weights) in LDS.
+
+#include <hip/hip_runtime.h>
+#include "hr_device.h"
+
+#pragma clang fp contract(off)
+
+namespace hr {
+namespace {
+
+constexpr int NT3 = 256;
+
+__device__ __forceinline__ double ldo(const double* p, unsigned off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + off);
+}
+__device__ __forceinline__ void sto(double* p, unsigned off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + off) = v;
+}
+
+__device__ __forceinline__ bool in_dom3(const Geo3& g, int i, int j, int k) {
+  return i >= 0 && j >= 0 && k >= 0 && i < g.NX && j < g.NY && k < g.NZ;
+}
+__device__ __forceinline__ bool u_act3(const Geo3& g, int i, int j, int k) {
+  return !(i == 0 || j == 0 || k == 0 || i == g.NX - 1 || j == g.NY - 1);
+}
+// node touched by at least one PML element
+__device__ __forceinline__ bool pml3(const Geo3& g, int i, int j, int k) {
+  return !(i > 2 * g.rlo[0] && i < 2 * g.rhi[0] && j > 2 * g.rlo[1] && j < 2 * g.rhi[1] && k > 2 * g.rlo[2]);
+}
+// active stress components (bit m) of a PML node
+__device__ __forceinline__ int smask3(const Geo3& g, int i, int j, int k) {
+  if (g.top_rule && k == g.NZ - 1 && !((i & 1) && (j & 1))) return 0x3f & ~((1 << 2) | (1 << 4) | (1 << 5));
+  return 0x3f;
+}
+__device__ __forceinline__ bool rd3(const Geo3& g, int ex, int ey, int ez) {
+  return ex >= g.rlo[0] && ex < g.rhi[0] && ey >= g.rlo[1] && ey < g.rhi[1] && ez >= g.rlo[2];
+}
+__device__ __forceinline__ int sym3(int i, int k) { return i == k ? i : i + k + 2; }
+
+// Stage-S inputs (v1, v2, v3) of one field component in the reference's nested
+// VecWAXPY order; v1 and v3 only where need13.  NEG: adjoint signs on v1, v3.
+template <int S, bool NEG>
+__device__ __forceinline__ void stage_in(const Stage3Args& A, unsigned off, bool need13, int mask,
+                                         double& v1, double& v2, double& v3) {
+  v1 = 0.0; v2 = 0.0; v3 = 0.0;
+  const double hh = A.hh, h = A.h;
+  if constexpr (S == 0) {
+    const double x = ldo(A.in[0], off);
+    if (mask & 1) v1 = x;
+    if (mask & 2) v2 = x;
+    if (mask & 4) v3 = x;
+    return;
+  } else if constexpr (S == 1) {
+    v2 = ldo(A.in[1], off);
+    if (need13) { v1 = ldo(A.in[0], off); v3 = ldo(A.in[2], off); }
+  } else if constexpr (S == 2) {
+    const double x2 = ldo(A.in[1], off), x3 = ldo(A.in[2], off);
+    v2 = x2 + hh * x3;
+    if (need13) { v1 = ldo(A.in[0], off) + hh * x2; v3 = x3 + hh * ldo(A.in[3], off); }
+  } else if constexpr (S == 3) {
+    const double x2 = ldo(A.in[1], off), x3 = ldo(A.in[2], off);
+    v2 = x2 + hh * (x3 + hh * ldo(A.in[3], off));
+    if (need13) { v1 = ldo(A.in[0], off) + hh * (x2 + hh * x3); v3 = x3 + hh * ldo(A.in[4], off); }
+  } else {
+    const double x2 = ldo(A.in[1], off), x3 = ldo(A.in[2], off);
+    v2 = x2 + h * (x3 + hh * ldo(A.in[4], off));
+    if (need13) {
+      v1 = ldo(A.in[0], off) + h * (x2 + hh * (x3 + hh * ldo(A.in[3], off)));
+      v3 = x3 + h * ldo(A.in[5], off);
+    }
+  }
+  if constexpr (NEG) { v1 = -v1; v3 = -v3; }
+}
+
+template <int TX, int TY>
+struct Shape3 {
+  static constexpr int WX = 2 * TX + 3, WY = 2 * TY + 3, WP = WX * WY;   // window (low-side halo element)
+  static constexpr int NEW = (TX + 1) * (TY + 1);                         // window elements
+  static constexpr int OX = 2 * TX + 1, OY = 2 * TY + 1, OC = OX * OY;    // owner columns
+  static constexpr int HP = WP - OC;                                      // halo positions per plane
+};
+
+template <int TX, int TY, bool HAS_RD, bool HAS_PML, int S, bool TR>
+__global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, const int* __restrict__ tiles, int ntl) {
+  using SH = Shape3<TX, TY>;
+  constexpr int WX = SH::WX, WP = SH::WP, NEW = SH::NEW, OX = SH::OX, OC = SH::OC, HP = SH::HP;
+  static_assert(3 * OC <= NT3, "three owner planes must fit the workgroup");
+  constexpr bool NEG = (S > 0) && TR;
+  constexpr int NS = (S == 4) ? (HAS_PML ? 9 : 3) : 1;   // stage-4 register carry width
+
+  __shared__ double sV[HAS_RD ? 3 : 1][HAS_RD ? 3 : 1][HAS_RD ? WP : 1];     // K input of u (v2)
+  __shared__ double sW[HAS_PML ? 3 : 1][HAS_PML ? 9 : 1][HAS_PML ? WP : 1];  // F-weighted u inputs, 3*d + i
+  __shared__ double sG[HAS_PML ? 3 : 1][HAS_PML ? 9 : 1][HAS_PML ? WP : 1];  // nodal PML flux, 3*i + k
+  __shared__ double sSig[HAS_RD ? NEW : 1][6][27];                            // regular domain W * sigma
+  __shared__ double sWq[27];
+  __shared__ double sDM[9];
+  __shared__ int sType[NEW];
+
+  const Geo3& g = A.g;
+  const int tid = threadIdx.x;
+  // XCD-aware list position: XCD x = blockIdx % 8 takes list entries [x*C, (x+1)*C)
+  const int C = (ntl + 7) / 8;
+  const int li = (blockIdx.x & 7) * C + (blockIdx.x >> 3);
+  if (li >= ntl) return;
+  const int b = tiles[li];
+  const int tx = b % g.tiles0;
+  const int ty = (b / g.tiles0) % g.tiles1;
+  const int tz = b / (g.tiles0 * g.tiles1);
+  const int ex0 = tx * TX, ey0 = ty * TY, ez0 = tz * g.tz;
+  const int ez_end = min(ez0 + g.tz, g.nez);
+  const int gi0 = 2 * ex0 - 2, gj0 = 2 * ey0 - 2;
+  const int own_i1 = (ex0 + TX >= g.nex) ? g.NX : 2 * (ex0 + TX);
+  const int own_j1 = (ey0 + TY >= g.ney) ? g.NY : 2 * (ey0 + TY);
+  const int ez_first = max(ez0 - 1, 0);
+  const int mask = (S == 0) ? A.set_mask : 7;
+  const bool useG = mask & 1, useK = mask & 2, useC = mask & 4;
+  const unsigned NN8 = (unsigned)g.NN * 8u;   // component stride in bytes
+  const double* __restrict__ tab = A.tab;
+  const double* __restrict__ nodet = A.node;
+
+  if (tid < 27) sWq[tid] = tab[tid];
+  if (tid < 9) {
+    const int a = tid / 3, c = tid - 3 * a;   // DM(a, c) = l_c'(xi_a)
+    sDM[tid] = (c == 0) ? (a == 0 ? -1.5 : (a == 1 ? -0.5 : 0.5))
+                        : (c == 1 ? (a == 0 ? 2.0 : (a == 1 ? 0.0 : -2.0)) : (a == 0 ? -0.5 : (a == 1 ? 0.5 : 1.5)));
+  }
+
+  // owner mapping: group P owns the lattice planes with k % 3 == P
+  const bool owner = tid < 3 * OC;
+  const int P = tid / OC, oc = tid - P * OC;
+  const int ix = oc % OX, iy = oc / OX;
+  const int i = 2 * ex0 + ix, j = 2 * ey0 + iy;
+  const bool owned = owner && i < own_i1 && j < own_j1;
+  const int rown = (ix + 2) + WX * (iy + 2);
+  const bool oddx = i & 1, oddy = j & 1;
+
+  // owner registers: accumulators, diagonal terms, stage-4 carry
+  double acc[9], dg[9], x3o[NS], pbo[NS];
+  double lam_o = 0.0, mu_o = 0.0;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) { acc[m] = 0.0; dg[m] = 0.0; }
+#pragma unroll
+  for (int m = 0; m < NS; ++m) { x3o[m] = 0.0; pbo[m] = 0.0; }
+
+  // Load one lattice node into the window ring; own: also the owner registers.
+  auto load = [&](int ii, int jj, int kk, int r, int slot, bool own, bool own_plane) {
+    double vu2[3] = {0.0, 0.0, 0.0};
+    double wv[9], Gv[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) { wv[m] = 0.0; Gv[m] = 0.0; }
+    if (in_dom3(g, ii, jj, kk)) {
+      const unsigned n = (unsigned)ii + (unsigned)g.NX * ((unsigned)jj + (unsigned)g.NY * (unsigned)kk);
+      const unsigned off = n * 8u;
+      const bool ua = u_act3(g, ii, jj, kk);
+      const bool pn = HAS_PML && pml3(g, ii, jj, kk);
+      double v1u[3] = {0.0, 0.0, 0.0}, v3u[3] = {0.0, 0.0, 0.0};
+      if (ua) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) stage_in<S, NEG>(A, off + c * NN8, pn, mask, v1u[c], vu2[c], v3u[c]);
+      }
+      if constexpr (S == 4) {
+        if (own && own_plane && ua) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const unsigned oc_ = off + c * NN8;
+            const double x1 = ldo(A.in[0], oc_), x2 = ldo(A.in[1], oc_), x3 = ldo(A.in[2], oc_);
+            const double g1 = ldo(A.in[3], oc_), g2 = ldo(A.in[4], oc_), g3 = ldo(A.in[5], oc_);
+            const double hh = A.hh, h = A.h, h6 = A.h6;
+            const double k2t = x2 + hh * x3, k3t = x2 + hh * (x3 + hh * g1), k4t = x2 + h * (x3 + hh * g2);
+            const double k2m = x3 + hh * g1, k3m = x3 + hh * g2, k4m = x3 + h * g3;
+            const double ht = 0.0 + (((1.0 * x2 + 2.0 * k2t) + 2.0 * k3t) + 1.0 * k4t);
+            const double hm = 0.0 + (((1.0 * x3 + 2.0 * k2m) + 2.0 * k3m) + 1.0 * k4m);
+            x3o[c] = x3;
+            pbo[c] = (1.0 * g1 + 2.0 * g2) + 2.0 * g3;
+            sto(A.xnew[0], oc_, x1 + h6 * ht);
+            sto(A.xnew[1], oc_, x2 + h6 * hm);
+          }
+        }
+      }
+      if constexpr (HAS_PML) {
+        if (pn) {
+          const unsigned ox = 8u * (unsigned)(TAB_STRETCH + ii), oy = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + jj);
+          const unsigned oz = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * g.NY + kk);
+          const double ax = ldo(tab, ox), bx = ldo(tab, ox + 8u * g.NX);
+          const double ay = ldo(tab, oy), by = ldo(tab, oy + 8u * g.NY);
+          const double az = ldo(tab, oz), bz = ldo(tab, oz + 8u * g.NZ);
+          // F[set][k], set 0 = w (G, v1), 1 = p (K, v2), 2 = e (C, v3)
+          double F[3][3];
+          F[2][0] = ay * az; F[2][1] = ax * az; F[2][2] = ax * ay;
+          F[1][0] = ay * bz + az * by; F[1][1] = ax * bz + az * bx; F[1][2] = ax * by + ay * bx;
+          F[0][0] = by * bz; F[0][1] = bx * bz; F[0][2] = bx * by;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              double s = 0.0;
+              if (useG) s += F[0][d] * v1u[c];
+              if (useK) s += F[1][d] * vu2[c];
+              if (useC) s += F[2][d] * v3u[c];
+              wv[3 * d + c] = s;
+            }
+          const int sm = smask3(g, ii, jj, kk);
+          double s1[6], s2[6], s3[6];
+#pragma unroll
+          for (int m = 0; m < 6; ++m) {
+            s1[m] = s2[m] = s3[m] = 0.0;
+            if (sm & (1 << m)) stage_in<S, NEG>(A, off + (3 + m) * NN8, true, mask, s1[m], s2[m], s3[m]);
+          }
+          const double lam = ldo(nodet, off + NT_LAM * NN8), mu = ldo(nodet, off + NT_MU * NN8);
+#pragma unroll
+          for (int X = 0; X < 3; ++X) {
+            if (!(mask & (1 << X))) continue;
+            const double* s = X == 0 ? s1 : (X == 1 ? s2 : s3);
+            if constexpr (!TR) {
+              // T(s): stress of the auxiliary strain (ELEMENT_EA_3D lower block)
+              const double l2m = lam + 2.0 * mu, m2 = 2.0 * mu;
+              const double Txx = l2m * s[0] + lam * s[1] + lam * s[2];
+              const double Tyy = lam * s[0] + l2m * s[1] + lam * s[2];
+              const double Tzz = lam * s[0] + lam * s[1] + l2m * s[2];
+              const double Txy = m2 * s[3], Txz = m2 * s[4], Tyz = m2 * s[5];
+              const double Tm[9] = {Txx, Txy, Txz, Txy, Tyy, Tyz, Txz, Tyz, Tzz};
+#pragma unroll
+              for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) Gv[3 * a + k] -= F[X][k] * Tm[3 * a + k];
+            } else {
+              const double Ym[9] = {s[0], s[3], s[4], s[3], s[1], s[5], s[4], s[5], s[2]};
+#pragma unroll
+              for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) Gv[3 * a + k] += F[X][k] * Ym[3 * a + k];
+            }
+          }
+          if (own) {
+            // diagonal terms rho W (b v3 + c v2 + d v1) of ELEMENT_EM/EN (PML.F90 items 7..10)
+            const double qb = ax * ay * bz + ax * az * by + ay * az * bx;
+            const double qc = ax * by * bz + ay * bx * bz + az * bx * by;
+            const double qd = bx * by * bz;
+            const double rwp = ldo(nodet, off + NT_RWP * NN8), wps = ldo(nodet, off + NT_WPS * NN8);
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {
+              double a1, a2, a3;
+              if (m < 3) { a1 = v1u[m]; a2 = vu2[m]; a3 = v3u[m]; }
+              else { a1 = s1[m - 3]; a2 = s2[m - 3]; a3 = s3[m - 3]; }
+              const double wm = m < 3 ? rwp : (m < 6 ? wps : 2.0 * wps);
+              double s = 0.0;
+              if (useC) s += qb * a3;
+              if (useK) s += qc * a2;
+              if (useG) s += qd * a1;
+              dg[m] = wm * s;
+            }
+            lam_o = lam;
+            mu_o = mu;
+            if constexpr (S == 4) {
+              if (own_plane) {
+#pragma unroll
+                for (int m = 3; m < 9; ++m) {
+                  if (!((sm >> (m - 3)) & 1)) continue;
+                  const unsigned oc_ = off + m * NN8;
+                  const double x1 = ldo(A.in[0], oc_), x2 = ldo(A.in[1], oc_);
+                  const double x3 = ldo(A.in[2], oc_);
+                  const double g1 = ldo(A.in[3], oc_), g2 = ldo(A.in[4], oc_);
+                  const double g3 = ldo(A.in[5], oc_);
+                  const double hh = A.hh, h = A.h, h6 = A.h6;
+                  const double k2t = x2 + hh * x3, k3t = x2 + hh * (x3 + hh * g1), k4t = x2 + h * (x3 + hh * g2);
+                  const double k2m = x3 + hh * g1, k3m = x3 + hh * g2, k4m = x3 + h * g3;
+                  const double ht = 0.0 + (((1.0 * x2 + 2.0 * k2t) + 2.0 * k3t) + 1.0 * k4t);
+                  const double hm = 0.0 + (((1.0 * x3 + 2.0 * k2m) + 2.0 * k3m) + 1.0 * k4m);
+                  if constexpr (NS == 9) {
+                    x3o[m] = x3;
+                    pbo[m] = (1.0 * g1 + 2.0 * g2) + 2.0 * g3;
+                  }
+                  sto(A.xnew[0], oc_, x1 + h6 * ht);
+                  sto(A.xnew[1], oc_, x2 + h6 * hm);
+                }
+              }
+            }
+          }
+        } else if (own) {
+#pragma unroll
+          for (int m = 0; m < 9; ++m) dg[m] = 0.0;
+        }
+      }
+    }
+    if constexpr (HAS_RD) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sV[slot][c][r] = vu2[c];
+    }
+    if constexpr (HAS_PML) {
+#pragma unroll
+      for (int m = 0; m < 9; ++m) { sW[slot][m][r] = wv[m]; sG[slot][m][r] = Gv[m]; }
+    }
+  };
+
+  for (int ez = ez_first; ez < ez_end; ++ez) {
+    const bool halo = ez < ez0;
+    const bool first = ez == ez_first;
+    const int kfirst = first ? 2 * ez : 2 * ez + 1;
+    const int p = ((P - 2 * ez) % 3 + 3) % 3;       // this group's plane within the slab
+    const int k = 2 * ez + p;
+    // ------------------------------------------------------------ loads
+    if (owner && k >= kfirst) {
+      const bool own_plane = k >= 2 * ez0 && (k < 2 * ez_end || k == g.NZ - 1);
+      load(i, j, k, rown, k % 3, owned, own_plane);
+    }
+    {
+      const int nnew = 2 * ez + 3 - kfirst;
+      int hr_, nh;
+      if (first) { hr_ = tid; nh = NT3; }
+      else {
+        const int idleP = (2 * ez) % 3;
+        hr_ = tid >= 3 * OC ? tid - 3 * OC : (P == idleP ? (NT3 - 3 * OC) + oc : -1);
+        nh = NT3 - 2 * OC;
+      }
+      if (hr_ >= 0) {
+        for (int hx = hr_; hx < nnew * HP; hx += nh) {
+          const int pl = hx / HP, q = hx - pl * HP;
+          int lx_, ly_;
+          if (q < 2 * WX) { ly_ = q / WX; lx_ = q - ly_ * WX; }
+          else { const int q2 = q - 2 * WX; ly_ = 2 + (q2 >> 1); lx_ = q2 & 1; }
+          const int kk = kfirst + pl;
+          load(gi0 + lx_, gj0 + ly_, kk, lx_ + WX * ly_, kk % 3, false, false);
+        }
+      }
+    }
+    for (int t = tid; t < NEW; t += NT3) {
+      const int ex = ex0 - 1 + t % (TX + 1), ey = ey0 - 1 + t / (TX + 1);
+      int ty_ = 0;
+      if (ex >= 0 && ey >= 0 && ex < g.nex && ey < g.ney) ty_ = rd3(g, ex, ey, ez) ? 1 : 2;
+      sType[t] = ty_;
+    }
+    __syncthreads();
+
+    // ------------------------------------------- regular-domain W * sigma
+    if constexpr (HAS_RD) {
+      if (useK) {
+        for (int t = tid; t < NEW * 27; t += NT3) {
+          const int e = t / 27, q = t - 27 * e;
+          if (sType[e] != 1) continue;
+          const int lx = q % 3, ly = (q / 3) % 3, lz = q / 9;
+          const int li0 = 2 * (e % (TX + 1)), lj0 = 2 * (e / (TX + 1));
+          const int rq = (li0 + lx) + WX * (lj0 + ly);
+          const double fx0 = sDM[3 * lx], fx1 = sDM[3 * lx + 1], fx2 = sDM[3 * lx + 2];
+          const double fy0 = sDM[3 * ly], fy1 = sDM[3 * ly + 1], fy2 = sDM[3 * ly + 2];
+          const double fz0 = sDM[3 * lz], fz1 = sDM[3 * lz + 1], fz2 = sDM[3 * lz + 2];
+          const int sq = (2 * ez + lz) % 3, z0 = (2 * ez) % 3, z1 = (2 * ez + 1) % 3, z2 = (2 * ez + 2) % 3;
+          double d[3][3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double* U = &sV[sq][c][0];
+            d[c][0] = g.dxi[0] * (fx0 * U[rq - lx] + fx1 * U[rq - lx + 1] + fx2 * U[rq - lx + 2]);
+            d[c][1] = g.dxi[1] * (fy0 * U[rq - WX * ly] + fy1 * U[rq - WX * ly + WX] + fy2 * U[rq - WX * ly + 2 * WX]);
+            d[c][2] = g.dxi[2] * (((0.0 + fz0 * sV[z0][c][rq]) + fz1 * sV[z1][c][rq]) + fz2 * sV[z2][c][rq]);
+          }
+          const unsigned n = (unsigned)(gi0 + li0 + lx) +
+                             (unsigned)g.NX * ((unsigned)(gj0 + lj0 + ly) + (unsigned)g.NY * (unsigned)(2 * ez + lz));
+          const double lam = ldo(nodet, n * 8u + NT_LAM * NN8), mu = ldo(nodet, n * 8u + NT_MU * NN8);
+          const double W = sWq[q];
+          const double tr = d[0][0] + d[1][1] + d[2][2];
+          sSig[e][0][q] = W * (lam * tr + 2.0 * mu * d[0][0]);
+          sSig[e][1][q] = W * (lam * tr + 2.0 * mu * d[1][1]);
+          sSig[e][2][q] = W * (lam * tr + 2.0 * mu * d[2][2]);
+          sSig[e][3][q] = W * (mu * (d[0][1] + d[1][0]));
+          sSig[e][4][q] = W * (mu * (d[0][2] + d[2][0]));
+          sSig[e][5][q] = W * (mu * (d[1][2] + d[2][1]));
+        }
+      }
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------- owner phase
+    if (p != 0 || first) {
+#pragma unroll
+      for (int m = 0; m < 9; ++m) acc[m] = 0.0;
+    }
+    const bool top_plane = (ez == g.nez - 1) && p == 2;
+    if (owned && (!halo || p == 2)) {
+      const int lz = p;
+      const int sP = k % 3;
+      const int z0 = (2 * ez) % 3, z1 = (2 * ez + 1) % 3, z2 = (2 * ez + 2) % 3;
+      const bool pn = HAS_PML && pml3(g, i, j, k);
+      // z coefficients: forward DM(lz, .), transposed DM(., lz)
+      const double fz0 = sDM[3 * lz], fz1 = sDM[3 * lz + 1], fz2 = sDM[3 * lz + 2];
+      const double tz0 = sDM[lz], tz1 = sDM[3 + lz], tz2 = sDM[6 + lz];
+      double aE[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+          int ex, lx, ey, ly;
+          if (oddx) { if (a) continue; ex = (i - 1) >> 1; lx = 1; }
+          else if (a == 0) { if (i < 2) continue; ex = (i >> 1) - 1; lx = 2; }
+          else { if ((i >> 1) >= g.nex) continue; ex = i >> 1; lx = 0; }
+          if (oddy) { if (bb) continue; ey = (j - 1) >> 1; ly = 1; }
+          else if (bb == 0) { if (j < 2) continue; ey = (j >> 1) - 1; ly = 2; }
+          else { if ((j >> 1) >= g.ney) continue; ey = j >> 1; ly = 0; }
+          const int e = (ex - ex0 + 1) + (TX + 1) * (ey - ey0 + 1);
+          const int et = sType[e];
+          const int rP = rown;                        // window position of (i, j)
+          const double tx0 = sDM[lx], tx1 = sDM[3 + lx], tx2 = sDM[6 + lx];
+          const double ty0 = sDM[ly], ty1 = sDM[3 + ly], ty2 = sDM[6 + ly];
+          if constexpr (HAS_RD) {
+            if (et == 1 && useK) {
+              const double* Se = &sSig[e][0][0];
+              const int bx_ = 3 * ly + 9 * lz, by_ = lx + 9 * lz, bz_ = lx + 3 * ly;
+#pragma unroll
+              for (int ii = 0; ii < 3; ++ii) {
+                const double* Sx = Se + 27 * sym3(ii, 0);
+                const double* Sy = Se + 27 * sym3(ii, 1);
+                const double* Sz = Se + 27 * sym3(ii, 2);
+                const double s = ((0.0 + tx0 * Sx[bx_]) + tx1 * Sx[bx_ + 1]) + tx2 * Sx[bx_ + 2];
+                const double t2 = ((0.0 + ty0 * Sy[by_]) + ty1 * Sy[by_ + 3]) + ty2 * Sy[by_ + 6];
+                const double t3 = ((0.0 + tz0 * Sz[bz_]) + tz1 * Sz[bz_ + 9]) + tz2 * Sz[bz_ + 18];
+                acc[ii] += g.dxi[0] * s + g.dxi[1] * t2 + g.dxi[2] * t3;
+              }
+            }
+          }
+          if constexpr (HAS_PML) {
+            if (et == 2) {
+              // u rows: transposed derivatives of the nodal flux, weights of the element points
+              const double wx0 = sWq[0 + 3 * ly + 9 * lz], wx1 = sWq[1 + 3 * ly + 9 * lz], wx2 = sWq[2 + 3 * ly + 9 * lz];
+              const double wy0 = sWq[lx + 0 + 9 * lz], wy1 = sWq[lx + 3 + 9 * lz], wy2 = sWq[lx + 6 + 9 * lz];
+              const double wz0 = sWq[lx + 3 * ly], wz1 = sWq[lx + 3 * ly + 9], wz2 = sWq[lx + 3 * ly + 18];
+              const int rx = rP - lx, ry = rP - WX * ly;
+#pragma unroll
+              for (int ii = 0; ii < 3; ++ii) {
+                const double* Gx = &sG[sP][3 * ii + 0][0];
+                const double* Gy = &sG[sP][3 * ii + 1][0];
+                const double s = ((0.0 + tx0 * (wx0 * Gx[rx])) + tx1 * (wx1 * Gx[rx + 1])) + tx2 * (wx2 * Gx[rx + 2]);
+                const double t2 =
+                    ((0.0 + ty0 * (wy0 * Gy[ry])) + ty1 * (wy1 * Gy[ry + WX])) + ty2 * (wy2 * Gy[ry + 2 * WX]);
+                const double t3 = ((0.0 + tz0 * (wz0 * sG[z0][3 * ii + 2][rP])) + tz1 * (wz1 * sG[z1][3 * ii + 2][rP])) +
+                                  tz2 * (wz2 * sG[z2][3 * ii + 2][rP]);
+                acc[ii] += g.dxi[0] * s + g.dxi[1] * t2 + g.dxi[2] * t3;
+              }
+              // s rows: F-weighted strains of the element at this point
+              const double fx0 = sDM[3 * lx], fx1 = sDM[3 * lx + 1], fx2 = sDM[3 * lx + 2];
+              const double fy0 = sDM[3 * ly], fy1 = sDM[3 * ly + 1], fy2 = sDM[3 * ly + 2];
+              double dX[3], dY[3], dZ[3];
+#pragma unroll
+              for (int ii = 0; ii < 3; ++ii) {
+                const double* Wx = &sW[sP][ii][0];
+                const double* Wy = &sW[sP][3 + ii][0];
+                dX[ii] = g.dxi[0] * (fx0 * Wx[rx] + fx1 * Wx[rx + 1] + fx2 * Wx[rx + 2]);
+                dY[ii] = g.dxi[1] * (fy0 * Wy[ry] + fy1 * Wy[ry + WX] + fy2 * Wy[ry + 2 * WX]);
+                dZ[ii] = g.dxi[2] * (((0.0 + fz0 * sW[z0][6 + ii][rP]) + fz1 * sW[z1][6 + ii][rP]) + fz2 * sW[z2][6 + ii][rP]);
+              }
+              const double W = sWq[lx + 3 * ly + 9 * lz];
+              aE[0] += W * dX[0];
+              aE[1] += W * dY[1];
+              aE[2] += W * dZ[2];
+              aE[3] += W * (dY[0] + dX[1]);
+              aE[4] += W * (dZ[0] + dX[2]);
+              aE[5] += W * (dZ[1] + dY[2]);
+            }
+          }
+        }
+      if (pn) {
+        if constexpr (!TR) {
+#pragma unroll
+          for (int m = 0; m < 6; ++m) acc[3 + m] += aE[m];
+        } else {
+          const double lam = lam_o, mu = mu_o;
+          const double l2m = lam + 2.0 * mu, m2 = 2.0 * mu;
+          acc[3] += -(l2m * aE[0] + lam * aE[1] + lam * aE[2]);
+          acc[4] += -(lam * aE[0] + l2m * aE[1] + lam * aE[2]);
+          acc[5] += -(lam * aE[0] + lam * aE[1] + l2m * aE[2]);
+          acc[6] += -(m2 * aE[3]);
+          acc[7] += -(m2 * aE[4]);
+          acc[8] += -(m2 * aE[5]);
+        }
+      }
+      // ------------------------------------------------------- finish
+      if (!(p == 2 && !top_plane) && !halo) {
+        const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
+        const unsigned off = n * 8u;
+        const bool ua = u_act3(g, i, j, k);
+        const int sm = pn ? smask3(g, i, j, k) : 0;
+        double raw[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) raw[m] = (HAS_PML && pn) ? acc[m] + dg[m] : acc[m];
+        if constexpr (S == 0) {
+#pragma unroll
+          for (int m = 0; m < 9; ++m) {
+            const bool act = m < 3 ? ua : ((sm >> (m - 3)) & 1);
+            if (act) sto(A.out, off + m * NN8, raw[m]);
+          }
+        } else {
+          const double minv_u = ua ? ldo(nodet, off + NT_MINV_U * NN8) : 0.0;
+          const double minv_s = sm ? ldo(nodet, off + NT_MINV_S * NN8) : 0.0;
+#pragma unroll
+          for (int m = 0; m < 9; ++m) {
+            const bool act = m < 3 ? ua : ((sm >> (m - 3)) & 1);
+            if (!act) continue;
+            if (!HAS_PML && m >= 3) continue;
+            double src = 0.0;
+            if (m < 3 && A.src_mode) {
+              const Src3* sp = A.src;
+              const int j0 = sp->tile_ptr[b], j1 = sp->tile_ptr[b + 1];
+              for (int jj = j0; jj < j1; ++jj) {
+                if (sp->node[jj] == (long long)n && sp->comp[jj] == m) {
+                  const double mm = sp->m[jj];
+                  double val;
+                  if (A.src_mode == 1) {
+                    val = A.ca * (sp->val[jj] * mm);
+                  } else {
+                    const size_t nr = (size_t)sp->nrec;
+                    const double va = sp->tr[(size_t)A.ra * nr + jj] - sp->obs[(size_t)A.ra * nr + jj];
+                    val = A.ca * (va * mm);
+                    if (A.rb >= 0) {
+                      const double vb = sp->tr[(size_t)A.rb * nr + jj] - sp->obs[(size_t)A.rb * nr + jj];
+                      val = val + A.cb * (vb * mm);
+                    }
+                  }
+                  src += val;
+                }
+              }
+            }
+            const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
+            const double gout = src - minv * raw[m];
+            if constexpr (S < 4) {
+              sto(A.out, off + m * NN8, gout);
+            } else {
+              const int cm = (NS == 9) ? m : (m < 3 ? m : 0);
+              sto(A.xnew[2], off + m * NN8, x3o[cm] + A.h6 * (0.0 + (pbo[cm] + 1.0 * gout)));
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int S, bool TR>
+hipError_t launch3(const Stage3Args& A, int cls, const int* tiles, int ntl, hipStream_t s) {
+  if (ntl <= 0) return hipSuccess;
+  const int nb = 8 * ((ntl + 7) / 8);
+  switch (cls) {
+    case 0: stage_kernel3<4, 3, true, true, S, TR><<<nb, NT3, 0, s>>>(A, tiles, ntl); break;
+    case 1: stage_kernel3<4, 3, true, false, S, TR><<<nb, NT3, 0, s>>>(A, tiles, ntl); break;
+    default: stage_kernel3<4, 3, false, true, S, TR><<<nb, NT3, 0, s>>>(A, tiles, ntl); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_stage3(const Stage3Args& A, int stage, int transpose, int cls, const int* tiles, int ntl,
+                         hipStream_t s) {
+  switch (stage * 2 + (transpose ? 1 : 0)) {
+    case 0: return launch3<0, false>(A, cls, tiles, ntl, s);
+    case 1: return launch3<0, true>(A, cls, tiles, ntl, s);
+    case 2: return launch3<1, false>(A, cls, tiles, ntl, s);
+    case 3: return launch3<1, true>(A, cls, tiles, ntl, s);
+    case 4: return launch3<2, false>(A, cls, tiles, ntl, s);
+    case 5: return launch3<2, true>(A, cls, tiles, ntl, s);
+    case 6: return launch3<3, false>(A, cls, tiles, ntl, s);
+    case 7: return launch3<3, true>(A, cls, tiles, ntl, s);
+    case 8: return launch3<4, false>(A, cls, tiles, ntl, s);
+    case 9: return launch3<4, true>(A, cls, tiles, ntl, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace hr
