@@ -33,6 +33,7 @@ def parse():
     p.add_argument("--tz", type=int, default=0)
     p.add_argument("--ty", type=int, default=0)
     p.add_argument("--version", type=int, default=0)
+    p.add_argument("--serial-classes", action="store_true", help="v3: run the tile classes one after another")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=12, help="oracle RK4 steps timed (each direction)")
     p.add_argument("--copy-calib", action="store_true", help="also run a plain 1 GiB device copy (PMC scale)")
@@ -103,7 +104,8 @@ def main():
     def T(x, dtype=torch.float64):
         return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=dev)
 
-    ctx = S.Context(mesh, device_ptrs=True, tz=a.tz, ty=a.ty, version=a.version)
+    version = a.version | (0x100 if a.serial_classes else 0)
+    ctx = S.Context(mesh, device_ptrs=True, tz=a.tz, ty=a.ty, version=version)
     d_ftab, d_src, d_val, d_rec = T(ftab), T(src_eq, torch.int64), T(src_val), T(rec_eq, torch.int64)
     d_obs = torch.zeros((nstep + 1, nrec), dtype=torch.float64, device=dev)
     d_tr = torch.zeros_like(d_obs)

@@ -20,7 +20,7 @@ using namespace hr;
 namespace {
 constexpr int DEFAULT_TZ = 8;
 constexpr int DEFAULT_TY = 3;
-constexpr int DEFAULT_VERSION = 2;
+constexpr int DEFAULT_VERSION = 3;
 
 struct HipError {
   hipError_t e;
@@ -98,6 +98,11 @@ struct hr_ctx {
   size_t obs_cap = 0, sorted_cap = 0;
   double recompute = 1.0;
   hipStream_t stream = nullptr;
+  // side streams (default priority) that run tile classes 1 and 2 of a stage
+  // concurrently with class 0 on the main stream; fork/join through events
+  hipStream_t cstream[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+  int concurrent = 1;
   // stage-kernel version and tile-class lists (v2)
   int version = DEFAULT_VERSION;
   int* d_tl[3] = {nullptr, nullptr, nullptr};
@@ -306,8 +311,20 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
       return fail(ctx, 3, "stage sign pattern not supported by the v3 kernel");
     if (A.nsrc_tiles && !(sp && sp->mode)) return fail(ctx, 3, "v3 stage launch without its source lists");
     const Stage3Args B = args3(ctx, A, sp);
-    for (int cls = 0; cls < 3; ++cls)
-      HIPC(launch_stage3(B, A.stage, A.transpose, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
+    if (ctx->concurrent) {
+      // classes own disjoint nodes: classes 1, 2 run on the side streams
+      HIPC(hipEventRecord(ctx->ev_fork, ctx->stream));
+      for (int q = 0; q < 2; ++q) {
+        HIPC(hipStreamWaitEvent(ctx->cstream[q], ctx->ev_fork, 0));
+        HIPC(launch_stage3(B, A.stage, A.transpose, q + 1, ctx->d_tl[q + 1], ctx->ntl[q + 1], ctx->cstream[q]));
+        HIPC(hipEventRecord(ctx->ev_join[q], ctx->cstream[q]));
+      }
+      HIPC(launch_stage3(B, A.stage, A.transpose, 0, ctx->d_tl[0], ctx->ntl[0], ctx->stream));
+      for (int q = 0; q < 2; ++q) HIPC(hipStreamWaitEvent(ctx->stream, ctx->ev_join[q], 0));
+    } else {
+      for (int cls = 0; cls < 3; ++cls)
+        HIPC(launch_stage3(B, A.stage, A.transpose, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
+    }
   } else {
     for (int cls = 0; cls < 3; ++cls) HIPC(launch_stage2(A, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
   }
@@ -521,6 +538,11 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
           if (h_active(g, i, j, k, c)) ctx->eq_slot.push_back((long long)c * g.NN + lat(g, i, j, k));
   ctx->neq = (long long)ctx->eq_slot.size();
   HIPC(hipStreamCreate(&ctx->stream));
+  for (int q = 0; q < 2; ++q) {
+    HIPC(hipStreamCreateWithFlags(&ctx->cstream[q], hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&ctx->ev_join[q], hipEventDisableTiming));
+  }
+  HIPC(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
   if (int rc = setup_tiles(ctx)) return rc;
   if (int rc = upload(ctx, &ctx->d_eq_slot, ctx->eq_slot)) return rc;
   // coordinates (origin: centred in x, y; top surface at z = 0)
@@ -565,6 +587,11 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   for (auto& p : ctx->ev_open) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  for (int q = 0; q < 2; ++q) {
+    if (ctx->cstream[q]) (void)hipStreamDestroy(ctx->cstream[q]);
+    if (ctx->ev_join[q]) (void)hipEventDestroy(ctx->ev_join[q]);
+  }
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   delete ctx;
   return 0;
 }
@@ -776,9 +803,12 @@ int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const doubl
 
 int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   if (!ctx) return 1;
+  const int serial = (version & HR_TILING_SERIAL_CLASSES) ? 1 : 0;
+  version &= ~HR_TILING_SERIAL_CLASSES;
   if (ty < 0 || tz < 0 || version < 0 || version > 3) return fail(ctx, 1, "bad tiling");
   Geo& g = ctx->g;
   const int v = version ? version : DEFAULT_VERSION;
+  ctx->concurrent = serial ? 0 : 1;
   const int t = ty ? ty : (v == 1 ? 4 : DEFAULT_TY);
   if (v == 1 && t != 4) return fail(ctx, 1, "stage kernel v1 tiles are 4 x 4 elements");
   if (v == 2 && t != 3 && t != 4) return fail(ctx, 1, "stage kernel v2 tiles are 4 x 3 or 4 x 4 elements");

@@ -28,9 +28,12 @@
 //    tables (GLL derivative matrix, weights) in LDS.
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "hr_device.h"
 
-#pragma clang fp contract(off)
+// Fused multiply-add is allowed here: the parity bar is a tolerance, and FMA
+// halves the FP64 instruction count of the stencils.
+#pragma clang fp contract(fast)
 
 namespace hr {
 namespace {
@@ -64,39 +67,48 @@ __device__ __forceinline__ bool rd3(const Geo3& g, int ex, int ey, int ez) {
 }
 __device__ __forceinline__ int sym3(int i, int k) { return i == k ? i : i + k + 2; }
 
-// Stage-S inputs (v1, v2, v3) of one field component in the reference's nested
-// VecWAXPY order; v1 and v3 only where need13.  NEG: adjoint signs on v1, v3.
+// Stage-S inputs (v1, v2, v3) of one field component from its raw inputs X
+// (x1 x2 x3 g1 g2 g3; apply: x) in the reference's nested VecWAXPY order.
+// NEG: adjoint signs on v1, v3.
 template <int S, bool NEG>
-__device__ __forceinline__ void stage_in(const Stage3Args& A, unsigned off, bool need13, int mask,
-                                         double& v1, double& v2, double& v3) {
-  v1 = 0.0; v2 = 0.0; v3 = 0.0;
-  const double hh = A.hh, h = A.h;
+__device__ __forceinline__ void combine(const double* X, double hh, double h, int mask, double& v1, double& v2,
+                                        double& v3) {
   if constexpr (S == 0) {
-    const double x = ldo(A.in[0], off);
-    if (mask & 1) v1 = x;
-    if (mask & 2) v2 = x;
-    if (mask & 4) v3 = x;
+    v1 = (mask & 1) ? X[0] : 0.0;
+    v2 = (mask & 2) ? X[0] : 0.0;
+    v3 = (mask & 4) ? X[0] : 0.0;
     return;
   } else if constexpr (S == 1) {
-    v2 = ldo(A.in[1], off);
-    if (need13) { v1 = ldo(A.in[0], off); v3 = ldo(A.in[2], off); }
+    v1 = X[0]; v2 = X[1]; v3 = X[2];
   } else if constexpr (S == 2) {
-    const double x2 = ldo(A.in[1], off), x3 = ldo(A.in[2], off);
-    v2 = x2 + hh * x3;
-    if (need13) { v1 = ldo(A.in[0], off) + hh * x2; v3 = x3 + hh * ldo(A.in[3], off); }
+    v1 = X[0] + hh * X[1]; v2 = X[1] + hh * X[2]; v3 = X[2] + hh * X[3];
   } else if constexpr (S == 3) {
-    const double x2 = ldo(A.in[1], off), x3 = ldo(A.in[2], off);
-    v2 = x2 + hh * (x3 + hh * ldo(A.in[3], off));
-    if (need13) { v1 = ldo(A.in[0], off) + hh * (x2 + hh * x3); v3 = x3 + hh * ldo(A.in[4], off); }
+    v1 = X[0] + hh * (X[1] + hh * X[2]); v2 = X[1] + hh * (X[2] + hh * X[3]); v3 = X[2] + hh * X[4];
   } else {
-    const double x2 = ldo(A.in[1], off), x3 = ldo(A.in[2], off);
-    v2 = x2 + h * (x3 + hh * ldo(A.in[4], off));
-    if (need13) {
-      v1 = ldo(A.in[0], off) + h * (x2 + hh * (x3 + hh * ldo(A.in[3], off)));
-      v3 = x3 + h * ldo(A.in[5], off);
-    }
+    v1 = X[0] + h * (X[1] + hh * (X[2] + hh * X[3])); v2 = X[1] + h * (X[2] + hh * X[4]); v3 = X[2] + h * X[5];
   }
   if constexpr (NEG) { v1 = -v1; v3 = -v3; }
+}
+
+// raw input q enters the K-set input v2 of stage S
+template <int S>
+__device__ __forceinline__ constexpr bool v2_input(int q) {
+  return S == 0 ? q == 0 : (S == 1 ? q == 1 : (S == 2 ? (q == 1 || q == 2) : (S == 3 ? (q >= 1 && q <= 3) : (q == 1 || q == 2 || q == 4))));
+}
+
+// Stage-4 update of x1 and x2 (no g4 needed), and the x3 carry
+// c3 = x3 + dt/6 (g1 + 2 g2 + 2 g3), so that x3_new = c3 + dt/6 g4
+// (Solver_PETSc_Explicit.F90:1036-1046)
+__device__ __forceinline__ void rk4_update12(const double* X, double hh, double h, double h6, double& xn0, double& xn1,
+                                             double& c3) {
+  const double x1 = X[0], x2 = X[1], x3v = X[2], g1 = X[3], g2 = X[4], g3 = X[5];
+  const double k2t = x2 + hh * x3v, k3t = x2 + hh * (x3v + hh * g1), k4t = x2 + h * (x3v + hh * g2);
+  const double k2m = x3v + hh * g1, k3m = x3v + hh * g2, k4m = x3v + h * g3;
+  const double ht = 0.0 + (((1.0 * x2 + 2.0 * k2t) + 2.0 * k3t) + 1.0 * k4t);
+  const double hm = 0.0 + (((1.0 * x3v + 2.0 * k2m) + 2.0 * k3m) + 1.0 * k4m);
+  xn0 = x1 + h6 * ht;
+  xn1 = x2 + h6 * hm;
+  c3 = x3v + h6 * ((1.0 * g1 + 2.0 * g2) + 2.0 * g3);
 }
 
 template <int TX, int TY>
@@ -108,7 +120,7 @@ struct Shape3 {
 };
 
 template <int TX, int TY, bool HAS_RD, bool HAS_PML, int S, bool TR>
-__global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, const int* __restrict__ tiles, int ntl) {
+__global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S <= 3 ? 3 : 2)) stage_kernel3(const Stage3Args A, const int* __restrict__ tiles, int ntl) {
   using SH = Shape3<TX, TY>;
   constexpr int WX = SH::WX, WP = SH::WP, NEW = SH::NEW, OX = SH::OX, OC = SH::OC, HP = SH::HP;
   static_assert(3 * OC <= NT3, "three owner planes must fit the workgroup");
@@ -118,6 +130,7 @@ __global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, cons
   __shared__ double sV[HAS_RD ? 3 : 1][HAS_RD ? 3 : 1][HAS_RD ? WP : 1];     // K input of u (v2)
   __shared__ double sW[HAS_PML ? 3 : 1][HAS_PML ? 9 : 1][HAS_PML ? WP : 1];  // F-weighted u inputs, 3*d + i
   __shared__ double sG[HAS_PML ? 3 : 1][HAS_PML ? 9 : 1][HAS_PML ? WP : 1];  // nodal PML flux, 3*i + k
+  __shared__ double sLM[HAS_RD ? 3 : 1][2][HAS_RD ? WP : 1];                  // lam, mu of the window nodes
   __shared__ double sSig[HAS_RD ? NEW : 1][6][27];                            // regular domain W * sigma
   __shared__ double sWq[27];
   __shared__ double sDM[9];
@@ -162,16 +175,23 @@ __global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, cons
   const bool oddx = i & 1, oddy = j & 1;
 
   // owner registers: accumulators, diagonal terms, stage-4 carry
-  double acc[9], dg[9], x3o[NS], pbo[NS];
-  double lam_o = 0.0, mu_o = 0.0;
+  double acc[9], dg[9], c3o[NS];
+  double lam_o = 0.0, mu_o = 0.0, minvu_o = 0.0, minvs_o = 0.0;
 #pragma unroll
   for (int m = 0; m < 9; ++m) { acc[m] = 0.0; dg[m] = 0.0; }
 #pragma unroll
-  for (int m = 0; m < NS; ++m) { x3o[m] = 0.0; pbo[m] = 0.0; }
+  for (int m = 0; m < NS; ++m) c3o[m] = 0.0;
 
-  // Load one lattice node into the window ring; own: also the owner registers.
-  auto load = [&](int ii, int jj, int kk, int r, int slot, bool own, bool own_plane) {
+  // Load one lattice node into the window ring; OWN: also the owner registers.
+  // The loads of a node class are issued unconditionally, so that they leave
+  // in one batch: inactive lattice entries are zero in every state vector and
+  // never written, and contribute nothing.  Activity only gates the stores.
+  constexpr int NIN = (S == 0) ? 1 : 2 + S;            // x1 x2 x3 g1 g2 g3 prefix (apply: x)
+  auto load = [&](auto own_tag, int ii, int jj, int kk, int r, int slot, bool own_plane) {
+    constexpr bool OWN = decltype(own_tag)::value;
+    constexpr bool ALL_U = HAS_PML || (OWN && S == 4);   // v1 and v3 of u, or the stage-4 update
     double vu2[3] = {0.0, 0.0, 0.0};
+    double lmv[2] = {0.0, 0.0};
     double wv[9], Gv[9];
 #pragma unroll
     for (int m = 0; m < 9; ++m) { wv[m] = 0.0; Gv[m] = 0.0; }
@@ -180,37 +200,54 @@ __global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, cons
       const unsigned off = n * 8u;
       const bool ua = u_act3(g, ii, jj, kk);
       const bool pn = HAS_PML && pml3(g, ii, jj, kk);
-      double v1u[3] = {0.0, 0.0, 0.0}, v3u[3] = {0.0, 0.0, 0.0};
-      if (ua) {
+      // ---- u components: raw inputs
+      double XU[3][NIN];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) stage_in<S, NEG>(A, off + c * NN8, pn, mask, v1u[c], vu2[c], v3u[c]);
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) {
+          const bool need = ALL_U || v2_input<S>(q);
+          XU[c][q] = need ? ldo(A.in[q], off + c * NN8) : 0.0;
+        }
+      if constexpr (HAS_RD) {
+        lmv[0] = ldo(nodet, off + NT_LAM * NN8);
+        lmv[1] = ldo(nodet, off + NT_MU * NN8);
       }
-      if constexpr (S == 4) {
-        if (own && own_plane && ua) {
+      if constexpr (OWN && S > 0) minvu_o = ldo(nodet, off + NT_MINV_U * NN8);
+      double v1u[3], v3u[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) combine<S, NEG>(XU[c], A.hh, A.h, mask, v1u[c], vu2[c], v3u[c]);
+      if constexpr (OWN && S == 4) {
+        if (own_plane && ua) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const unsigned oc_ = off + c * NN8;
-            const double x1 = ldo(A.in[0], oc_), x2 = ldo(A.in[1], oc_), x3 = ldo(A.in[2], oc_);
-            const double g1 = ldo(A.in[3], oc_), g2 = ldo(A.in[4], oc_), g3 = ldo(A.in[5], oc_);
-            const double hh = A.hh, h = A.h, h6 = A.h6;
-            const double k2t = x2 + hh * x3, k3t = x2 + hh * (x3 + hh * g1), k4t = x2 + h * (x3 + hh * g2);
-            const double k2m = x3 + hh * g1, k3m = x3 + hh * g2, k4m = x3 + h * g3;
-            const double ht = 0.0 + (((1.0 * x2 + 2.0 * k2t) + 2.0 * k3t) + 1.0 * k4t);
-            const double hm = 0.0 + (((1.0 * x3 + 2.0 * k2m) + 2.0 * k3m) + 1.0 * k4m);
-            x3o[c] = x3;
-            pbo[c] = (1.0 * g1 + 2.0 * g2) + 2.0 * g3;
-            sto(A.xnew[0], oc_, x1 + h6 * ht);
-            sto(A.xnew[1], oc_, x2 + h6 * hm);
+            double xn0, xn1;
+            rk4_update12(XU[c], A.hh, A.h, A.h6, xn0, xn1, c3o[c]);
+            sto(A.xnew[0], off + c * NN8, xn0);
+            sto(A.xnew[1], off + c * NN8, xn1);
           }
         }
       }
       if constexpr (HAS_PML) {
-        if (pn) {
+        // Branch-free over the node class: at a node outside the PML the stress
+        // components are zero (so G = 0), b_x = b_y = b_z = 0 (so the diagonal
+        // terms vanish), and w is never read (PML elements touch PML nodes only).
+        {
+          // ---- stretch factors, material
           const unsigned ox = 8u * (unsigned)(TAB_STRETCH + ii), oy = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + jj);
           const unsigned oz = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * g.NY + kk);
           const double ax = ldo(tab, ox), bx = ldo(tab, ox + 8u * g.NX);
           const double ay = ldo(tab, oy), by = ldo(tab, oy + 8u * g.NY);
           const double az = ldo(tab, oz), bz = ldo(tab, oz + 8u * g.NZ);
+          double lam, mu;
+          if constexpr (HAS_RD) { lam = lmv[0]; mu = lmv[1]; }
+          else { lam = ldo(nodet, off + NT_LAM * NN8); mu = ldo(nodet, off + NT_MU * NN8); }
+          double rwp = 0.0, wps = 0.0;
+          if constexpr (OWN) {
+            rwp = ldo(nodet, off + NT_RWP * NN8);
+            wps = ldo(nodet, off + NT_WPS * NN8);
+            if constexpr (S > 0) minvs_o = pn ? ldo(nodet, off + NT_MINV_S * NN8) : 0.0;
+          }
           // F[set][k], set 0 = w (G, v1), 1 = p (K, v2), 2 = e (C, v3)
           double F[3][3];
           F[2][0] = ay * az; F[2][1] = ax * az; F[2][2] = ax * ay;
@@ -220,98 +257,96 @@ __global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, cons
           for (int d = 0; d < 3; ++d)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-              double s = 0.0;
-              if (useG) s += F[0][d] * v1u[c];
-              if (useK) s += F[1][d] * vu2[c];
-              if (useC) s += F[2][d] * v3u[c];
-              wv[3 * d + c] = s;
+              double sacc = 0.0;
+              if (useG) sacc += F[0][d] * v1u[c];
+              if (useK) sacc += F[1][d] * vu2[c];
+              if (useC) sacc += F[2][d] * v3u[c];
+              wv[3 * d + c] = sacc;
             }
-          const int sm = smask3(g, ii, jj, kk);
-          double s1[6], s2[6], s3[6];
+          // diagonal terms rho W (b v3 + c v2 + d v1) of ELEMENT_EM/EN (PML.F90 items 7..10)
+          const double qb = ax * ay * bz + ax * az * by + ay * az * bx;
+          const double qc = ax * by * bz + ay * bx * bz + az * bx * by;
+          const double qd = bx * by * bz;
+          auto diag = [&](double wm, double a1, double a2, double a3) {
+            double sacc = 0.0;
+            if (useC) sacc += qb * a3;
+            if (useK) sacc += qc * a2;
+            if (useG) sacc += qd * a1;
+            return wm * sacc;
+          };
+          if constexpr (OWN) {
 #pragma unroll
-          for (int m = 0; m < 6; ++m) {
-            s1[m] = s2[m] = s3[m] = 0.0;
-            if (sm & (1 << m)) stage_in<S, NEG>(A, off + (3 + m) * NN8, true, mask, s1[m], s2[m], s3[m]);
-          }
-          const double lam = ldo(nodet, off + NT_LAM * NN8), mu = ldo(nodet, off + NT_MU * NN8);
-#pragma unroll
-          for (int X = 0; X < 3; ++X) {
-            if (!(mask & (1 << X))) continue;
-            const double* s = X == 0 ? s1 : (X == 1 ? s2 : s3);
-            if constexpr (!TR) {
-              // T(s): stress of the auxiliary strain (ELEMENT_EA_3D lower block)
-              const double l2m = lam + 2.0 * mu, m2 = 2.0 * mu;
-              const double Txx = l2m * s[0] + lam * s[1] + lam * s[2];
-              const double Tyy = lam * s[0] + l2m * s[1] + lam * s[2];
-              const double Tzz = lam * s[0] + lam * s[1] + l2m * s[2];
-              const double Txy = m2 * s[3], Txz = m2 * s[4], Tyz = m2 * s[5];
-              const double Tm[9] = {Txx, Txy, Txz, Txy, Tyy, Tyz, Txz, Tyz, Tzz};
-#pragma unroll
-              for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) Gv[3 * a + k] -= F[X][k] * Tm[3 * a + k];
-            } else {
-              const double Ym[9] = {s[0], s[3], s[4], s[3], s[1], s[5], s[4], s[5], s[2]};
-#pragma unroll
-              for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) Gv[3 * a + k] += F[X][k] * Ym[3 * a + k];
-            }
-          }
-          if (own) {
-            // diagonal terms rho W (b v3 + c v2 + d v1) of ELEMENT_EM/EN (PML.F90 items 7..10)
-            const double qb = ax * ay * bz + ax * az * by + ay * az * bx;
-            const double qc = ax * by * bz + ay * bx * bz + az * bx * by;
-            const double qd = bx * by * bz;
-            const double rwp = ldo(nodet, off + NT_RWP * NN8), wps = ldo(nodet, off + NT_WPS * NN8);
-#pragma unroll
-            for (int m = 0; m < 9; ++m) {
-              double a1, a2, a3;
-              if (m < 3) { a1 = v1u[m]; a2 = vu2[m]; a3 = v3u[m]; }
-              else { a1 = s1[m - 3]; a2 = s2[m - 3]; a3 = s3[m - 3]; }
-              const double wm = m < 3 ? rwp : (m < 6 ? wps : 2.0 * wps);
-              double s = 0.0;
-              if (useC) s += qb * a3;
-              if (useK) s += qc * a2;
-              if (useG) s += qd * a1;
-              dg[m] = wm * s;
-            }
+            for (int c = 0; c < 3; ++c) dg[c] = diag(rwp, v1u[c], vu2[c], v3u[c]);
             lam_o = lam;
             mu_o = mu;
-            if constexpr (S == 4) {
-              if (own_plane) {
+          }
+          const double l2m = lam + 2.0 * mu, m2 = 2.0 * mu;
+          const int sm = smask3(g, ii, jj, kk);
+          // ---- stress components in two halves (normal, shear): G splits the same way
 #pragma unroll
-                for (int m = 3; m < 9; ++m) {
-                  if (!((sm >> (m - 3)) & 1)) continue;
-                  const unsigned oc_ = off + m * NN8;
-                  const double x1 = ldo(A.in[0], oc_), x2 = ldo(A.in[1], oc_);
-                  const double x3 = ldo(A.in[2], oc_);
-                  const double g1 = ldo(A.in[3], oc_), g2 = ldo(A.in[4], oc_);
-                  const double g3 = ldo(A.in[5], oc_);
-                  const double hh = A.hh, h = A.h, h6 = A.h6;
-                  const double k2t = x2 + hh * x3, k3t = x2 + hh * (x3 + hh * g1), k4t = x2 + h * (x3 + hh * g2);
-                  const double k2m = x3 + hh * g1, k3m = x3 + hh * g2, k4m = x3 + h * g3;
-                  const double ht = 0.0 + (((1.0 * x2 + 2.0 * k2t) + 2.0 * k3t) + 1.0 * k4t);
-                  const double hm = 0.0 + (((1.0 * x3 + 2.0 * k2m) + 2.0 * k3m) + 1.0 * k4m);
-                  if constexpr (NS == 9) {
-                    x3o[m] = x3;
-                    pbo[m] = (1.0 * g1 + 2.0 * g2) + 2.0 * g3;
+          for (int hf = 0; hf < 2; ++hf) {
+            double XS[3][NIN];
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+              for (int q = 0; q < NIN; ++q) XS[t][q] = ldo(A.in[q], off + (3 + 3 * hf + t) * NN8);
+            double s1[3], s2[3], s3[3];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) combine<S, NEG>(XS[t], A.hh, A.h, mask, s1[t], s2[t], s3[t]);
+#pragma unroll
+            for (int X = 0; X < 3; ++X) {
+              if (!(mask & (1 << X))) continue;
+              const double* sv = X == 0 ? s1 : (X == 1 ? s2 : s3);
+              if (hf == 0) {
+                if constexpr (!TR) {
+                  // T(s): stress of the auxiliary strain (ELEMENT_EA_3D lower block), normal part
+                  const double Txx = l2m * sv[0] + lam * sv[1] + lam * sv[2];
+                  const double Tyy = lam * sv[0] + l2m * sv[1] + lam * sv[2];
+                  const double Tzz = lam * sv[0] + lam * sv[1] + l2m * sv[2];
+                  Gv[0] -= F[X][0] * Txx; Gv[4] -= F[X][1] * Tyy; Gv[8] -= F[X][2] * Tzz;
+                } else {
+                  Gv[0] += F[X][0] * sv[0]; Gv[4] += F[X][1] * sv[1]; Gv[8] += F[X][2] * sv[2];
+                }
+              } else {
+                if constexpr (!TR) {
+                  const double Txy = m2 * sv[0], Txz = m2 * sv[1], Tyz = m2 * sv[2];
+                  Gv[1] -= F[X][1] * Txy; Gv[2] -= F[X][2] * Txz; Gv[3] -= F[X][0] * Txy;
+                  Gv[5] -= F[X][2] * Tyz; Gv[6] -= F[X][0] * Txz; Gv[7] -= F[X][1] * Tyz;
+                } else {
+                  Gv[1] += F[X][1] * sv[0]; Gv[2] += F[X][2] * sv[1]; Gv[3] += F[X][0] * sv[0];
+                  Gv[5] += F[X][2] * sv[2]; Gv[6] += F[X][0] * sv[1]; Gv[7] += F[X][1] * sv[2];
+                }
+              }
+            }
+            if constexpr (OWN) {
+              const double wm = hf == 0 ? wps : 2.0 * wps;
+#pragma unroll
+              for (int t = 0; t < 3; ++t) dg[3 + 3 * hf + t] = diag(wm, s1[t], s2[t], s3[t]);
+              if constexpr (S == 4) {
+                if (own_plane) {
+#pragma unroll
+                  for (int t = 0; t < 3; ++t) {
+                    const int m = 3 * hf + t;
+                    double xn0, xn1, c3v;
+                    rk4_update12(XS[t], A.hh, A.h, A.h6, xn0, xn1, c3v);
+                    if constexpr (NS == 9) c3o[3 + m] = c3v;
+                    if ((sm >> m) & 1) {
+                      sto(A.xnew[0], off + (3 + m) * NN8, xn0);
+                      sto(A.xnew[1], off + (3 + m) * NN8, xn1);
+                    }
                   }
-                  sto(A.xnew[0], oc_, x1 + h6 * ht);
-                  sto(A.xnew[1], oc_, x2 + h6 * hm);
                 }
               }
             }
           }
-        } else if (own) {
-#pragma unroll
-          for (int m = 0; m < 9; ++m) dg[m] = 0.0;
         }
       }
     }
     if constexpr (HAS_RD) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) sV[slot][c][r] = vu2[c];
+      sLM[slot][0][r] = lmv[0];
+      sLM[slot][1][r] = lmv[1];
     }
     if constexpr (HAS_PML) {
 #pragma unroll
@@ -328,7 +363,8 @@ __global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, cons
     // ------------------------------------------------------------ loads
     if (owner && k >= kfirst) {
       const bool own_plane = k >= 2 * ez0 && (k < 2 * ez_end || k == g.NZ - 1);
-      load(i, j, k, rown, k % 3, owned, own_plane);
+      if (owned) load(std::true_type{}, i, j, k, rown, k % 3, own_plane);
+      else load(std::false_type{}, i, j, k, rown, k % 3, false);
     }
     {
       const int nnew = 2 * ez + 3 - kfirst;
@@ -346,7 +382,7 @@ __global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, cons
           if (q < 2 * WX) { ly_ = q / WX; lx_ = q - ly_ * WX; }
           else { const int q2 = q - 2 * WX; ly_ = 2 + (q2 >> 1); lx_ = q2 & 1; }
           const int kk = kfirst + pl;
-          load(gi0 + lx_, gj0 + ly_, kk, lx_ + WX * ly_, kk % 3, false, false);
+          load(std::false_type{}, gi0 + lx_, gj0 + ly_, kk, lx_ + WX * ly_, kk % 3, false);
         }
       }
     }
@@ -379,9 +415,7 @@ __global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, cons
             d[c][1] = g.dxi[1] * (fy0 * U[rq - WX * ly] + fy1 * U[rq - WX * ly + WX] + fy2 * U[rq - WX * ly + 2 * WX]);
             d[c][2] = g.dxi[2] * (((0.0 + fz0 * sV[z0][c][rq]) + fz1 * sV[z1][c][rq]) + fz2 * sV[z2][c][rq]);
           }
-          const unsigned n = (unsigned)(gi0 + li0 + lx) +
-                             (unsigned)g.NX * ((unsigned)(gj0 + lj0 + ly) + (unsigned)g.NY * (unsigned)(2 * ez + lz));
-          const double lam = ldo(nodet, n * 8u + NT_LAM * NN8), mu = ldo(nodet, n * 8u + NT_MU * NN8);
+          const double lam = sLM[sq][0][rq], mu = sLM[sq][1][rq];
           const double W = sWq[q];
           const double tr = d[0][0] + d[1][1] + d[2][2];
           sSig[e][0][q] = W * (lam * tr + 2.0 * mu * d[0][0]);
@@ -513,8 +547,7 @@ __global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, cons
             if (act) sto(A.out, off + m * NN8, raw[m]);
           }
         } else {
-          const double minv_u = ua ? ldo(nodet, off + NT_MINV_U * NN8) : 0.0;
-          const double minv_s = sm ? ldo(nodet, off + NT_MINV_S * NN8) : 0.0;
+          const double minv_u = minvu_o, minv_s = minvs_o;
 #pragma unroll
           for (int m = 0; m < 9; ++m) {
             const bool act = m < 3 ? ua : ((sm >> (m - 3)) & 1);
@@ -549,7 +582,7 @@ __global__ void __launch_bounds__(NT3, 2) stage_kernel3(const Stage3Args A, cons
               sto(A.out, off + m * NN8, gout);
             } else {
               const int cm = (NS == 9) ? m : (m < 3 ? m : 0);
-              sto(A.xnew[2], off + m * NN8, x3o[cm] + A.h6 * (0.0 + (pbo[cm] + 1.0 * gout)));
+              sto(A.xnew[2], off + m * NN8, c3o[cm] + A.h6 * gout);
             }
           }
         }

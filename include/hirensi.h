@@ -120,8 +120,11 @@ double hr_recompute_ratio(const hr_ctx* ctx);
 
 /* Stage-kernel configuration: tile of 4 x ty elements in x-y, tz elements
  * per z-chunk, kernel version (1: single-class tiles of 4 x 4; 2: tile
- * classes, ty = 3 or 4).  0 keeps a default.  Invalidates the history of the
- * last hr_forward. */
+ * classes, ty = 3 or 4; 3: tile classes with per-stage specialisation,
+ * ty = 3).  0 keeps a default.  Version 3 runs the tile classes of a stage
+ * concurrently on two side streams unless HR_TILING_SERIAL_CLASSES is or-ed
+ * into version.  Invalidates the history of the last hr_forward. */
+#define HR_TILING_SERIAL_CLASSES 0x100
 int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version);
 
 /* Stage-kernel timing with HIP events recorded around every stage-kernel
