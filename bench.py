@@ -35,7 +35,7 @@ def parse():
     p.add_argument("--version", type=int, default=0)
     p.add_argument("--serial-classes", action="store_true", help="v3: run the tile classes one after another")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-steps", type=int, default=12, help="oracle RK4 steps timed (each direction)")
+    p.add_argument("--cpu-steps", type=int, default=40, help="oracle RK4 steps timed (each direction)")
     p.add_argument("--copy-calib", action="store_true", help="also run a plain 1 GiB device copy (PMC scale)")
     return p.parse_args()
 
@@ -50,6 +50,16 @@ def stage_bytes(neq, nnode):
         writes = neq if s < 4 else 3 * neq
         b[s] = 8.0 * (reads + writes)
     return b
+
+
+def measured_traffic():
+    """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "stage_kernel_traffic_64x64x32.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def cpu_baseline(cfg, nsteps):
@@ -155,6 +165,7 @@ def main():
     alg_bytes = sum(sb[s] * n_l[s] for s in (1, 2, 3, 4))
     avg_us = kern_ms * 1e3 / launches
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    tr = measured_traffic()
     out = {
         "metric": "fwd+adjoint gradient DOF-steps/s (64x64x32, 4000 steps, 1 GPU)",
         "value": value,
@@ -173,7 +184,9 @@ def main():
                    "recompute_ratio": ctx.recompute_ratio(), "misfit": misfit, "g_mu_norm": gnorm,
                    "gradient_wall_s": wall, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": tr["traffic_bytes_per_launch"] if tr else None,
+                     "traffic_source": "profiles/stage_kernel_traffic_64x64x32.json" if tr else None,
                      "kernel": "stage kernel (all tile classes of one RK4 stage)", "launches": launches,
                      "avg_launch_us": avg_us,
                      "alg_bytes_per_launch": alg_bytes / launches,
