@@ -105,6 +105,7 @@ struct hr_ctx {
   int concurrent = 1;
   // stage-kernel version and tile-class lists (v2)
   int version = DEFAULT_VERSION;
+  std::vector<int> zb;                      // z-chunk boundaries (element layers)
   int* d_tl[3] = {nullptr, nullptr, nullptr};
   int ntl[3] = {0, 0, 0};
   // stage-kernel timing
@@ -185,10 +186,13 @@ long long lat(const Geo& g, int i, int j, int k) {
   return (long long)i + (long long)g.NX * ((long long)j + (long long)g.NY * k);
 }
 
-int tile_of(const Geo& g, int i, int j, int k) {
+// tile owning lattice node (i, j, k): chunk c owns planes [2 zb[c], 2 zb[c+1]), the last one also the top
+int tile_of(const hr_ctx* ctx, int i, int j, int k) {
+  const Geo& g = ctx->g;
   const int tx = std::min(i / (2 * g.tx), g.tiles[0] - 1);
   const int ty = std::min(j / (2 * g.ty), g.tiles[1] - 1);
-  const int tz = std::min(k / (2 * g.tz), g.tiles[2] - 1);
+  int tz = 0;
+  while (tz + 1 < g.tiles[2] && k >= 2 * ctx->zb[tz + 1]) ++tz;
   return tx + g.tiles[0] * (ty + g.tiles[1] * tz);
 }
 
@@ -211,7 +215,7 @@ SrcBuild build_src(hr_ctx* ctx, int n, const long long* eq) {
     const int c = (int)(slot / g.NN);
     const long long nd = slot - c * g.NN;
     const int i = (int)(nd % g.NX), j = (int)((nd / g.NX) % g.NY), k = (int)(nd / ((long long)g.NX * g.NY));
-    key[q] = {tile_of(g, i, j, k), q};
+    key[q] = {tile_of(ctx, i, j, k), q};
   }
   std::stable_sort(key.begin(), key.end());
   SrcBuild s;
@@ -275,6 +279,7 @@ Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
   B.g.rhi[0] = g.rhi[0]; B.g.rhi[1] = g.rhi[1];
   B.g.top_rule = g.top_rule;
   B.g.tiles0 = g.tiles[0]; B.g.tiles1 = g.tiles[1]; B.g.tz = g.tz;
+  for (size_t c = 0; c < ctx->zb.size() && c <= (size_t)HR_MAX_ZCHUNKS; ++c) B.g.zb[c] = ctx->zb[c];
   for (int a = 0; a < 3; ++a) B.g.dxi[a] = g.dxi[a];
   for (int q = 0; q < 6; ++q) B.in[q] = A.in[q];
   B.out = A.out;
@@ -352,13 +357,20 @@ int setup_tiles(hr_ctx* ctx) {
   Geo& g = ctx->g;
   g.tiles[0] = (g.nex + g.tx - 1) / g.tx;
   g.tiles[1] = (g.ney + g.ty - 1) / g.ty;
-  g.tiles[2] = (g.nez + g.tz - 1) / g.tz;
+  // z-chunks of tz element layers (the v3 kernel reads the boundaries, so chunks could differ;
+  // aligning one with the bottom PML interface measured slower: PML-only slabs cost as much)
+  std::vector<int>& zb = ctx->zb;
+  zb.assign(1, 0);
+  for (int z = g.tz; z < g.nez; z += g.tz) zb.push_back(z);
+  zb.push_back(g.nez);
+  if ((int)zb.size() - 1 > HR_MAX_ZCHUNKS) return fail(ctx, 1, "too many z-chunks (%d); raise tz", (int)zb.size() - 1);
+  g.tiles[2] = (int)zb.size() - 1;
   std::vector<int> lists[3];
   const int nt = ntiles(g);
   for (int b = 0; b < nt; ++b) {
     const int tx = b % g.tiles[0], ty = (b / g.tiles[0]) % g.tiles[1], tz = b / (g.tiles[0] * g.tiles[1]);
     bool has_rd = false, has_pml = false;
-    for (int ez = std::max(tz * g.tz - 1, 0); ez < std::min((tz + 1) * g.tz, g.nez); ++ez)
+    for (int ez = std::max(zb[tz] - 1, 0); ez < zb[tz + 1]; ++ez)
       for (int ey = std::max(ty * g.ty - 1, 0); ey < std::min((ty + 1) * g.ty, g.ney); ++ey)
         for (int ex = std::max(tx * g.tx - 1, 0); ex < std::min((tx + 1) * g.tx, g.nex); ++ex) {
           const bool rd = ex >= g.rlo[0] && ex < g.rhi[0] && ey >= g.rlo[1] && ey < g.rhi[1] && ez >= g.rlo[2];
@@ -366,6 +378,8 @@ int setup_tiles(hr_ctx* ctx) {
         }
     lists[(has_rd && has_pml) ? 0 : (has_rd ? 1 : 2)].push_back(b);
   }
+  // Class lists stay in tile order (x fastest): each XCD (blockIdx mod 8) takes a contiguous
+  // run of rows.  A Morton order of the runs measured more fabric traffic.
   for (int c = 0; c < 3; ++c) {
     ctx->ntl[c] = (int)lists[c].size();
     if (int rc = upload(ctx, &ctx->d_tl[c], lists[c])) return rc;

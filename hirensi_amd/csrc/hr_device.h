@@ -84,6 +84,7 @@ hipError_t launch_misfit(int nstep, int nrec, const double* tr, const double* ob
 
 namespace hr {
 // ----------------------------------------------------------- stage kernel v3
+constexpr int HR_MAX_ZCHUNKS = 64;
 // Compact launch arguments: per-node and per-axis coefficients live in device
 // tables, source lists behind a device pointer.
 struct Geo3 {
@@ -93,6 +94,7 @@ struct Geo3 {
   int top_rule;
   int tiles0, tiles1, tz;
   double dxi[3];
+  int zb[HR_MAX_ZCHUNKS + 1];      // z-chunk c spans element layers [zb[c], zb[c+1])
 };
 
 // Node table: [lam | mu | minv_u | minv_s | rwp | wps], NN entries each.

@@ -146,8 +146,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   const int tx = b % g.tiles0;
   const int ty = (b / g.tiles0) % g.tiles1;
   const int tz = b / (g.tiles0 * g.tiles1);
-  const int ex0 = tx * TX, ey0 = ty * TY, ez0 = tz * g.tz;
-  const int ez_end = min(ez0 + g.tz, g.nez);
+  const int ex0 = tx * TX, ey0 = ty * TY, ez0 = g.zb[tz];
+  const int ez_end = g.zb[tz + 1];
   const int gi0 = 2 * ex0 - 2, gj0 = 2 * ey0 - 2;
   const int own_i1 = (ex0 + TX >= g.nex) ? g.NX : 2 * (ex0 + TX);
   const int own_j1 = (ey0 + TY >= g.ney) ? g.NY : 2 * (ey0 + TY);
