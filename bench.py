@@ -37,6 +37,10 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=40, help="oracle RK4 steps timed (each direction)")
     p.add_argument("--copy-calib", action="store_true", help="also run a plain 1 GiB device copy (PMC scale)")
+    p.add_argument("--shots", type=int, default=0,
+                   help="instead of the default line: batched multi-shot gradient on the 48x48x24 survey "
+                        "(use with --nstep; shots/s against single-shot solves)")
+    p.add_argument("--seq-shots", type=int, default=2, help="single-shot gradients timed for the comparison")
     p.add_argument("--fd-check", action="store_true",
                    help="instead of timing: check the lambda/mu gradient against central finite differences "
                         "along 5 seeded smooth directions on this mesh (use with --nstep)")
@@ -145,6 +149,8 @@ def fd_check(a):
                     mp.mu, mm.mu = base + h * d, base - h * d
                 fd = (misfit(mp) - misfit(mm)) / (2.0 * h)
                 errs.append(abs(adj - fd) / abs(fd))
+                print(f"fd-check direction {t} {field} h/|m| {hr:g}: rel. err. {errs[-1]:.3e}", file=sys.stderr,
+                      flush=True)
             rows.append({"direction": t, "field": field, "adjoint": adj, "rel_err": errs})
     print(json.dumps({"metric": "gradient vs central finite differences (64x64x32, PML 6)",
                       "nstep": nstep, "misfit": j0, "recompute_ratio": ctx.recompute_ratio(),
@@ -153,8 +159,87 @@ def fd_check(a):
     ctx.close()
 
 
+def shots_bench(a):
+    """Batched multi-shot forward + adjoint + gradient of the 48x48x24 survey:
+    all shots advance together (one stage-kernel launch per RK4 stage covers
+    every shot), checkpointed history.  The same shots solved one at a time
+    through the single-shot path give the comparison (a bounded sample of
+    --seq-shots solves, per-shot time extrapolated).  Prints one JSON line."""
+    import numpy as np
+    import torch
+    from hirensi_amd import solver as S
+    from hirensi_amd.mesh import config3, receiver_dofs
+    cfg = config3(nshot=a.shots)
+    mesh, dt = cfg["mesh"], cfg["dt"]
+    nstep = a.nstep if a.nstep != 4000 else cfg["nstep"]
+    nshot = a.shots
+    dev = torch.device("cuda:0")
+    rec_eq = receiver_dofs(mesh, cfg["rec_nodes"])
+    nrec = len(rec_eq)
+    src_eq = np.array([mesh.dof(nd, 2) for nd in cfg["src_nodes"]], dtype=np.int64)
+    src_shot = np.arange(nshot, dtype=np.int32)
+    ftab = np.stack([S.time_table(nstep, dt, 2, fr) for fr in cfg["frs"]])
+    seg = 128
+
+    def T(x, dtype=torch.float64):
+        return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=dev)
+
+    ctx = S.Context(mesh, device_ptrs=True)
+    d_ftab, d_shot, d_src, d_val, d_rec = (T(ftab), T(src_shot, torch.int32), T(src_eq, torch.int64),
+                                           T(np.ones(nshot)), T(rec_eq, torch.int64))
+    d_obs = torch.zeros((nshot, nstep + 1, nrec), dtype=torch.float64, device=dev)
+    d_tr = torch.zeros_like(d_obs)
+    d_gl = torch.zeros(mesh.nnode, dtype=torch.float64, device=dev)
+    d_gm = torch.zeros_like(d_gl)
+    d_mis = torch.zeros(1, dtype=torch.float64, device=dev)
+    tm, sm = cfg["true"], cfg["start"]
+    ctx.set_material_ptrs(T(tm.lam), T(tm.mu), T(tm.rho))
+    S.explicit_rk4_petsc_pml_3d_shots(ctx, nstep, dt, d_ftab, d_shot, d_src, d_val, d_rec,
+                                      history=S.HR_HISTORY_NONE, traces=d_obs, nshot=nshot)
+    ctx.set_material_ptrs(T(sm.lam), T(sm.mu), T(sm.rho))
+    torch.cuda.synchronize()
+    print(f"shots: observed data of {nshot} shots done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    S.explicit_rk4_petsc_pml_3d_shots(ctx, nstep, dt, d_ftab, d_shot, d_src, d_val, d_rec,
+                                      history=S.HR_HISTORY_CHECKPOINT, segment=seg, traces=d_tr, nshot=nshot)
+    torch.cuda.synchronize()
+    print(f"shots: batched forward done ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, d_rec, d_obs, d_gl, d_gm, misfit_out=d_mis)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    print(f"shots: batched gradient done ({t_batch:.1f} s)", file=sys.stderr, flush=True)
+    ratio = ctx.recompute_ratio()
+    misfit = float(d_mis.item())
+    # single shots through the same library, one at a time
+    nseq = max(1, min(a.seq_shots, nshot))
+    d_tr1 = torch.zeros((nstep + 1, nrec), dtype=torch.float64, device=dev)
+    t0 = time.perf_counter()
+    for q in range(nseq):
+        S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, d_ftab[q], d_src[q:q + 1], d_val[q:q + 1], d_rec,
+                                    history=S.HR_HISTORY_CHECKPOINT, segment=seg, traces=d_tr1)
+        S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, d_rec, d_obs[q], d_gl, d_gm, misfit_out=d_mis)
+        torch.cuda.synchronize()
+        print(f"shots: single shot {q} done", file=sys.stderr, flush=True)
+    t_one = (time.perf_counter() - t0) / nseq
+    ctx.close()
+    dof_steps = ctx.neq * 2.0 * nstep * nshot
+    print(json.dumps({
+        "metric": f"batched multi-shot gradient shots/s (48x48x24, {nshot} shots, {nstep} steps, 1 GPU)",
+        "value": nshot / t_batch, "unit": "shots/s", "higher_is_better": True, "n_gpus": 1,
+        "dtype": "f64", "data": "synthetic (layered model, anomaly in the observed-data model)",
+        "config": {"workload": f"fwd+adj+grad of {nshot} shots, 48x48x24 el, PML 6, {nstep} RK4 steps, "
+                               f"{nrec // 3} shared receivers, checkpoint segment {seg}",
+                   "neq": ctx.neq, "recompute_ratio": ratio, "misfit_sum": misfit},
+        "batched_wall_s": t_batch, "batched_dof_steps_per_s": dof_steps / t_batch,
+        "single_shot_wall_s": t_one, "single_shot_sample": nseq,
+        "sequential_shots_per_s": 1.0 / t_one, "speedup_vs_sequential": t_one * nshot / t_batch}), flush=True)
+
+
 def main():
     a = parse()
+    if a.shots:
+        shots_bench(a)
+        return
     if a.fd_check:
         fd_check(a)
         return

@@ -29,7 +29,7 @@ struct HipError {
 // per-launch source parameters of the v3 stage kernel
 struct Src3Spec {
   const Src3* dev = nullptr;
-  int mode = 0, ra = 0, rb = -1;
+  int mode = 0, ra = 0, rb = -1, step = 0, fcol = 0;
   double ca = 0.0, cb = 0.0;
 };
 
@@ -67,8 +67,12 @@ struct hr_ctx {
   double* tmp = nullptr;                    // lattice scratch for hr_apply
   double* tmp2 = nullptr;
   Shell shell[3];
+  // shots held in the state buffers (shot-major, 9 NN per shot and vector)
+  int nshot = 1, shot_cap = 1;
+  double* d_ftab = nullptr;                 // time-function table of the last forward (nshot x nstep x 3)
+  size_t ftab_cap = 0;
   // forward record
-  int f_nstep = 0, f_nrec = 0, f_history = 0, f_segment = 0;
+  int f_nshot = 1, f_nstep = 0, f_nrec = 0, f_history = 0, f_segment = 0;
   double f_dt = 0.0;
   double* d_traces = nullptr;
   size_t traces_cap = 0;
@@ -117,6 +121,8 @@ struct hr_ctx {
 };
 
 namespace {
+
+size_t field_size(const hr_ctx* ctx) { return (size_t)NCOMP * ctx->g.NN; }
 
 int fail(hr_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -290,7 +296,11 @@ Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
   B.hh = A.hh; B.h = A.h; B.h6 = A.h6;
   if (sp && sp->mode) {
     B.src = sp->dev; B.src_mode = sp->mode; B.ra = sp->ra; B.rb = sp->rb; B.ca = sp->ca; B.cb = sp->cb;
+    B.step = sp->step; B.fcol = sp->fcol;
   }
+  // stage launches over the state buffers cover every shot; hr_apply works on one field
+  B.nshot = A.stage > 0 ? ctx->nshot : 1;
+  B.shot_stride = (long long)field_size(ctx);
   return B;
 }
 
@@ -303,6 +313,8 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
     if (!e0 || !e1) return fail(ctx, 3, "hipEventCreate failed");
     HIPC(hipEventRecord(e0, ctx->stream));
   }
+  if (ctx->version != 3 && A.stage > 0 && ctx->nshot != 1)
+    return fail(ctx, 3, "batched shots need the v3 stage kernel");
   if (ctx->version == 1) {
     HIPC(launch_stage(A, ntiles(ctx->g), ctx->stream));
   } else if (ctx->version == 3) {
@@ -398,8 +410,23 @@ int check_ready(hr_ctx* ctx) {
   return 0;
 }
 
+// state buffers for nshot shots (shot-major); contents are not kept on growth
+int ensure_shots(hr_ctx* ctx, int nshot) {
+  const size_t nf = field_size(ctx);
+  if (nshot > ctx->shot_cap) {
+    for (int a = 0; a < 3; ++a) {
+      if (int rc = dalloc(ctx, &ctx->x[a], nf * nshot)) return rc;
+      if (int rc = dalloc(ctx, &ctx->y[a], nf * nshot)) return rc;
+      if (int rc = dalloc(ctx, &ctx->gs[a], nf * nshot)) return rc;
+    }
+    ctx->shot_cap = nshot;
+  }
+  ctx->nshot = nshot;
+  return 0;
+}
+
 int zero_state(hr_ctx* ctx) {
-  const size_t n = (size_t)NCOMP * ctx->g.NN;
+  const size_t n = field_size(ctx) * ctx->nshot;
   for (int a = 0; a < 3; ++a) {
     HIPC(hipMemsetAsync(ctx->x[a], 0, n * sizeof(double), ctx->stream));
     HIPC(hipMemsetAsync(ctx->y[a], 0, n * sizeof(double), ctx->stream));
@@ -408,7 +435,7 @@ int zero_state(hr_ctx* ctx) {
   return 0;
 }
 
-// One RK4 step on (x) -> (y), then swap.  dir = +1 forward, -1 adjoint.
+// One RK4 step of every shot on (x) -> (y), then swap.  dir = +1 forward, -1 adjoint.
 int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3], const Source* src4,
              const Src3Spec* sp4) {
   const double hh = dir * (0.5 * dt), h = dir * dt, h6 = dir * (dt / 6.0);
@@ -430,48 +457,57 @@ int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3]
   return 0;
 }
 
+// receiver traces of every shot at the start of step n
 int record_step(hr_ctx* ctx, int n) {
-  if (ctx->f_nrec > 0) {
-    HIPC(launch_record(ctx->f_nrec, ctx->d_rec_slot, ctx->x[1], ctx->d_traces + (size_t)n * ctx->f_nrec, ctx->stream));
-  }
+  if (ctx->f_nrec <= 0) return 0;
+  const size_t nf = field_size(ctx), rows = (size_t)ctx->f_nstep + 1;
+  for (int q = 0; q < ctx->nshot; ++q)
+    HIPC(launch_record(ctx->f_nrec, ctx->d_rec_slot, ctx->x[1] + q * nf,
+                       ctx->d_traces + ((size_t)q * rows + n) * ctx->f_nrec, ctx->stream));
   return 0;
 }
 
-int store_slice(hr_ctx* ctx, double* dst) {
-  HIPC(launch_store_box(ctx->g, ctx->box0, ctx->boxn, ctx->x[1], dst, ctx->stream));
+// regular-domain displacement of every shot into history row `row` of blocks of `rows` rows per shot
+int store_slice(hr_ctx* ctx, double* base, size_t rows, size_t row) {
+  const size_t nf = field_size(ctx), slice = 3 * (size_t)ctx->nbox;
+  for (int q = 0; q < ctx->nshot; ++q)
+    HIPC(launch_store_box(ctx->g, ctx->box0, ctx->boxn, ctx->x[1] + q * nf, base + ((size_t)q * rows + row) * slice,
+                          ctx->stream));
   return 0;
 }
 
-// forward sources of step n: a = B values, ca = f
+// forward sources of step n: B values times f(T), f(T + dt/2), f(T + dt)
 void forward_sources(hr_ctx* ctx, int n, Source s4[4], Src3Spec sp4[4]) {
   for (int s = 0; s < 4; ++s) {
     std::memset(&s4[s], 0, sizeof(Source));
     sp4[s] = Src3Spec();
     if (!ctx->d_src_tile_ptr) continue;
+    const int col = (s == 0) ? 0 : (s == 3 ? 2 : 1);
+    // single-shot v1/v2 kernels take f from the host table
     s4[s].tile_ptr = ctx->d_src_tile_ptr;
     s4[s].node = ctx->d_src_node;
     s4[s].comp = ctx->d_src_comp;
     s4[s].m = ctx->d_src_m;
     s4[s].a = ctx->d_src_val;
-    const double* f = &ctx->ftab[3 * (size_t)n];
-    s4[s].ca = (s == 0) ? f[0] : (s == 3 ? f[2] : f[1]);
+    s4[s].ca = ctx->ftab[3 * (size_t)n + col];
     sp4[s].dev = ctx->d_src3_fwd;
     sp4[s].mode = 1;
-    sp4[s].ca = s4[s].ca;
+    sp4[s].step = n;
+    sp4[s].fcol = col;
   }
 }
 
-int run_forward_steps(hr_ctx* ctx, int n0, int n1, double* seg_hist) {
+int run_forward_steps(hr_ctx* ctx, int n0, int n1, double* seg_hist, size_t seg_rows) {
   // steps n0 .. n1-1 from the current state; stores slices n0..n1 into seg_hist when given
   const double sgn[3] = {1.0, 1.0, 1.0};
   for (int n = n0; n < n1; ++n) {
-    if (seg_hist) if (int rc = store_slice(ctx, seg_hist + (size_t)(n - n0) * 3 * ctx->nbox)) return rc;
+    if (seg_hist) if (int rc = store_slice(ctx, seg_hist, seg_rows, (size_t)(n - n0))) return rc;
     Source s4[4];
     Src3Spec sp4[4];
     forward_sources(ctx, n, s4, sp4);
     if (int rc = rk4_step(ctx, ctx->f_dt, +1, 0, sgn, s4, sp4)) return rc;
   }
-  if (seg_hist) if (int rc = store_slice(ctx, seg_hist + (size_t)(n1 - n0) * 3 * ctx->nbox)) return rc;
+  if (seg_hist) if (int rc = store_slice(ctx, seg_hist, seg_rows, (size_t)(n1 - n0))) return rc;
   return 0;
 }
 
@@ -588,7 +624,7 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(ctx->d_eq_slot); F(ctx->d_xc); F(ctx->d_yc); F(ctx->d_zc); F(ctx->d_lam); F(ctx->d_mu); F(ctx->d_minv);
   F(ctx->d_rwp); F(ctx->d_wps);
-  F(ctx->d_node); F(ctx->d_tab); F(ctx->d_src3_fwd); F(ctx->d_src3_adj);
+  F(ctx->d_node); F(ctx->d_tab); F(ctx->d_src3_fwd); F(ctx->d_src3_adj); F(ctx->d_ftab);
   for (int a = 0; a < 3; ++a) { F(ctx->x[a]); F(ctx->y[a]); F(ctx->gs[a]); F(ctx->shell[a].l); F(ctx->shell[a].r); }
   F(ctx->tmp); F(ctx->tmp2); F(ctx->d_traces); F(ctx->d_rec_slot); F(ctx->d_hist); F(ctx->d_seg);
   for (double* p : ctx->ckpt) F(p);
@@ -834,67 +870,102 @@ int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   return setup_tiles(ctx);
 }
 
-int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, const long long* src_eq,
-               const double* src_val, int nrec, const long long* rec_eq, double* traces, int history, int segment) {
+int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double* ftab, int nsrc, const int* src_shot,
+                     const long long* src_eq, const double* src_val, int nrec, const long long* rec_eq, double* traces,
+                     int history, int segment) {
   if (int rc = check_ready(ctx)) return rc;
+  if (nshot < 1) return fail(ctx, 1, "nshot must be >= 1");
+  if (nshot > 1 && ctx->version != 3) return fail(ctx, 1, "batched shots need the v3 stage kernel");
   if (nstep < 1 || !(dt > 0.0)) return fail(ctx, 1, "nstep must be >= 1 and dt > 0");
   if (!ftab) return fail(ctx, 1, "null time-function table");
   if (nsrc < 0 || nrec < 0) return fail(ctx, 1, "negative counts");
+  if (nsrc > 0 && (!src_eq || !src_val || (nshot > 1 && !src_shot))) return fail(ctx, 1, "null source arrays");
+  if (nrec > 0 && !rec_eq) return fail(ctx, 1, "null receiver array");
   if (history < 0 || history > 2) return fail(ctx, 1, "bad history mode");
   if (history == HR_HISTORY_CHECKPOINT && segment < 1) return fail(ctx, 1, "checkpoint segment must be >= 1");
   const bool dev = ctx->flags & HR_DEVICE_PTRS;
   // host copies of the small index arrays
   std::vector<long long> seq(nsrc), req(nrec);
   std::vector<double> sval(nsrc);
+  std::vector<int> sshot(nsrc, 0);
   if (nsrc) {
     if (dev) {
       HIPC(hipMemcpy(seq.data(), src_eq, nsrc * sizeof(long long), hipMemcpyDeviceToHost));
       HIPC(hipMemcpy(sval.data(), src_val, nsrc * sizeof(double), hipMemcpyDeviceToHost));
+      if (src_shot) HIPC(hipMemcpy(sshot.data(), src_shot, nsrc * sizeof(int), hipMemcpyDeviceToHost));
     } else {
       std::memcpy(seq.data(), src_eq, nsrc * sizeof(long long));
       std::memcpy(sval.data(), src_val, nsrc * sizeof(double));
+      if (src_shot) std::memcpy(sshot.data(), src_shot, nsrc * sizeof(int));
     }
   }
   if (nrec) {
     if (dev) HIPC(hipMemcpy(req.data(), rec_eq, nrec * sizeof(long long), hipMemcpyDeviceToHost));
     else std::memcpy(req.data(), rec_eq, nrec * sizeof(long long));
   }
-  for (int q = 0; q < nsrc; ++q)
+  for (int q = 0; q < nsrc; ++q) {
     if (seq[q] < 0 || seq[q] >= ctx->neq) return fail(ctx, 1, "source equation %lld out of range", seq[q]);
+    if (ctx->eq_slot[seq[q]] / ctx->g.NN >= 3) return fail(ctx, 1, "source equation %lld is not a displacement", seq[q]);
+    if (sshot[q] < 0 || sshot[q] >= nshot) return fail(ctx, 1, "source shot %d out of range", sshot[q]);
+  }
   for (int q = 0; q < nrec; ++q) {
     if (req[q] < 0 || req[q] >= ctx->neq) return fail(ctx, 1, "receiver equation %lld out of range", req[q]);
     if (ctx->eq_slot[req[q]] / ctx->g.NN >= 3) return fail(ctx, 1, "receiver equation %lld is not a displacement", req[q]);
   }
-  ctx->ftab.resize(3 * (size_t)nstep);
-  if (dev) HIPC(hipMemcpy(ctx->ftab.data(), ftab, 3 * (size_t)nstep * sizeof(double), hipMemcpyDeviceToHost));
-  else std::memcpy(ctx->ftab.data(), ftab, 3 * (size_t)nstep * sizeof(double));
-  // sources grouped by tile
+  const size_t nft = 3 * (size_t)nstep * nshot;
+  ctx->ftab.resize(nft);
+  if (dev) HIPC(hipMemcpy(ctx->ftab.data(), ftab, nft * sizeof(double), hipMemcpyDeviceToHost));
+  else std::memcpy(ctx->ftab.data(), ftab, nft * sizeof(double));
+  if (ctx->ftab_cap < nft) {
+    if (int rc = dalloc(ctx, &ctx->d_ftab, nft)) return rc;
+    ctx->ftab_cap = nft;
+  }
+  HIPC(hipMemcpy(ctx->d_ftab, ctx->ftab.data(), nft * sizeof(double), hipMemcpyHostToDevice));
+  if (int rc = ensure_shots(ctx, nshot)) return rc;
+  // sources grouped by shot, then tile
   if (nsrc) {
-    for (int q = 0; q < nsrc; ++q)
-      if (ctx->eq_slot[seq[q]] / ctx->g.NN >= 3) return fail(ctx, 1, "source equation %lld is not a displacement", seq[q]);
-    SrcBuild sb = build_src(ctx, nsrc, seq.data());
-    std::vector<double> v(nsrc);
-    for (int q = 0; q < nsrc; ++q) v[q] = sval[sb.perm[q]];
-    if (int rc = upload(ctx, &ctx->d_src_tile_ptr, sb.ptr)) return rc;
-    if (int rc = upload(ctx, &ctx->d_src_node, sb.node)) return rc;
-    if (int rc = upload(ctx, &ctx->d_src_comp, sb.comp)) return rc;
-    if (int rc = upload(ctx, &ctx->d_src_m, sb.m)) return rc;
+    const int nt = ntiles(ctx->g);
+    std::vector<int> ptr(1, 0);
+    std::vector<long long> node;
+    std::vector<int> comp;
+    std::vector<double> m, v;
+    for (int q = 0; q < nshot; ++q) {
+      std::vector<long long> eq;
+      std::vector<double> val;
+      for (int e = 0; e < nsrc; ++e)
+        if (sshot[e] == q) { eq.push_back(seq[e]); val.push_back(sval[e]); }
+      SrcBuild sb = build_src(ctx, (int)eq.size(), eq.data());
+      const int base = (int)node.size();
+      for (int t = 0; t < nt; ++t) ptr.push_back(base + sb.ptr[t + 1]);
+      for (size_t e = 0; e < eq.size(); ++e) {
+        node.push_back(sb.node[e]);
+        comp.push_back(sb.comp[e]);
+        m.push_back(sb.m[e]);
+        v.push_back(val[sb.perm[e]]);
+      }
+    }
+    if (int rc = upload(ctx, &ctx->d_src_tile_ptr, ptr)) return rc;
+    if (int rc = upload(ctx, &ctx->d_src_node, node)) return rc;
+    if (int rc = upload(ctx, &ctx->d_src_comp, comp)) return rc;
+    if (int rc = upload(ctx, &ctx->d_src_m, m)) return rc;
     if (int rc = upload(ctx, &ctx->d_src_val, v)) return rc;
     Src3 h3;
     std::memset(&h3, 0, sizeof(h3));
     h3.tile_ptr = ctx->d_src_tile_ptr; h3.node = ctx->d_src_node; h3.comp = ctx->d_src_comp;
-    h3.m = ctx->d_src_m; h3.val = ctx->d_src_val;
+    h3.m = ctx->d_src_m; h3.val = ctx->d_src_val; h3.ftab = ctx->d_ftab; h3.nstep = nstep; h3.ntiles = nt;
     if (int rc = upload(ctx, &ctx->d_src3_fwd, std::vector<Src3>{h3})) return rc;
   } else if (ctx->d_src_tile_ptr) {
     (void)hipFree(ctx->d_src_tile_ptr);
     ctx->d_src_tile_ptr = nullptr;
   }
   // receivers
-  ctx->f_nstep = nstep; ctx->f_dt = dt; ctx->f_nrec = nrec; ctx->f_history = history; ctx->f_segment = segment;
+  ctx->f_nshot = nshot; ctx->f_nstep = nstep; ctx->f_dt = dt; ctx->f_nrec = nrec; ctx->f_history = history;
+  ctx->f_segment = segment;
   ctx->rec_slot_h.resize(nrec);
   for (int q = 0; q < nrec; ++q) ctx->rec_slot_h[q] = ctx->eq_slot[req[q]];
   if (int rc = upload(ctx, &ctx->d_rec_slot, ctx->rec_slot_h)) return rc;
-  const size_t ntr = (size_t)(nstep + 1) * (nrec > 0 ? nrec : 1);
+  const size_t rows = (size_t)nstep + 1;
+  const size_t ntr = (size_t)nshot * rows * (nrec > 0 ? nrec : 1);
   if (ctx->traces_cap < ntr) {
     if (int rc = dalloc(ctx, &ctx->d_traces, ntr)) return rc;
     ctx->traces_cap = ntr;
@@ -904,7 +975,7 @@ int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, 
   for (double* p : ctx->ckpt) (void)hipFree(p);
   ctx->ckpt.clear();
   if (history == HR_HISTORY_STORE) {
-    const size_t need = (size_t)(nstep + 1) * slice;
+    const size_t need = (size_t)nshot * rows * slice;
     if (ctx->hist_cap < need) {
       if (int rc = dalloc(ctx, &ctx->d_hist, need)) return rc;
       ctx->hist_cap = need;
@@ -912,16 +983,16 @@ int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, 
   }
   if (int rc = zero_state(ctx)) return rc;
   const double sgn[3] = {1.0, 1.0, 1.0};
-  const size_t nf = (size_t)NCOMP * ctx->g.NN;
+  const size_t nfs = field_size(ctx) * nshot;
   for (int n = 0; n < nstep; ++n) {
     if (int rc = record_step(ctx, n)) return rc;
-    if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist + (size_t)n * slice)) return rc;
+    if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist, rows, (size_t)n)) return rc;
     if (history == HR_HISTORY_CHECKPOINT && n % segment == 0) {
       double* cp = nullptr;
-      if (int rc = dalloc(ctx, &cp, 3 * nf)) return rc;
+      if (int rc = dalloc(ctx, &cp, 3 * nfs)) return rc;
       ctx->ckpt.push_back(cp);
       for (int a = 0; a < 3; ++a)
-        HIPC(hipMemcpyAsync(cp + a * nf, ctx->x[a], nf * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        HIPC(hipMemcpyAsync(cp + a * nfs, ctx->x[a], nfs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     }
     Source s4[4];
     Src3Spec sp4[4];
@@ -929,21 +1000,47 @@ int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, 
     if (int rc = rk4_step(ctx, dt, +1, 0, sgn, s4, sp4)) return rc;
   }
   if (int rc = record_step(ctx, nstep)) return rc;
-  if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist + (size_t)nstep * slice)) return rc;
+  if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist, rows, (size_t)nstep)) return rc;
   if (traces && nrec) {
-    HIPC(hipMemcpyAsync(traces, ctx->d_traces, (size_t)(nstep + 1) * nrec * sizeof(double),
+    HIPC(hipMemcpyAsync(traces, ctx->d_traces, (size_t)nshot * rows * nrec * sizeof(double),
                         dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
   }
   HIPC(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
+int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, const long long* src_eq,
+               const double* src_val, int nrec, const long long* rec_eq, double* traces, int history, int segment) {
+  return hr_forward_shots(ctx, 1, nstep, dt, ftab, nsrc, nullptr, src_eq, src_val, nrec, rec_eq, traces, history,
+                          segment);
+}
+
+}  // extern "C"
+
+namespace {
+// misfit of every shot of the last forward against observations on the device, summed over
+// shots in shot order into *out (device)
+int misfit_all(hr_ctx* ctx, double dt, const double* dobs, double* out) {
+  const int nshot = ctx->f_nshot, nstep = ctx->f_nstep, nrec = ctx->f_nrec;
+  const size_t rows = (size_t)nstep + 1;
+  if (int rc = dalloc(ctx, &ctx->d_part, (size_t)nshot * rows + nshot)) return rc;
+  double* per_shot = ctx->d_part + (size_t)nshot * rows;
+  for (int q = 0; q < nshot; ++q)
+    HIPC(launch_misfit(nstep, nrec, ctx->d_traces + (size_t)q * rows * nrec, dobs + (size_t)q * rows * nrec, dt,
+                       ctx->d_part + (size_t)q * rows, per_shot + q, ctx->stream));
+  HIPC(launch_sum(nshot, per_shot, out, ctx->stream));
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_meas, double* misfit) {
   if (int rc = check_ready(ctx)) return rc;
   if (nstep != ctx->f_nstep || nrec != ctx->f_nrec) return fail(ctx, 1, "nstep/nrec differ from the last forward");
   if (!dis_meas || !misfit) return fail(ctx, 1, "null argument");
   const bool dev = ctx->flags & HR_DEVICE_PTRS;
-  const size_t nobs = (size_t)(nstep + 1) * nrec;
+  const size_t nobs = (size_t)ctx->f_nshot * (nstep + 1) * nrec;
   const double* dobs = dis_meas;
   if (!dev) {
     if (ctx->obs_cap < nobs) {
@@ -953,9 +1050,8 @@ int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_mea
     HIPC(hipMemcpyAsync(ctx->d_obs, dis_meas, nobs * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     dobs = ctx->d_obs;
   }
-  if (int rc = dalloc(ctx, &ctx->d_part, nstep + 1)) return rc;
   if (!ctx->d_scalar) if (int rc = dalloc(ctx, &ctx->d_scalar, 1)) return rc;
-  HIPC(launch_misfit(nstep, nrec, ctx->d_traces, dobs, dt, ctx->d_part, ctx->d_scalar, ctx->stream));
+  if (int rc = misfit_all(ctx, dt, dobs, ctx->d_scalar)) return rc;
   HIPC(hipMemcpyAsync(misfit, ctx->d_scalar, sizeof(double), dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                       ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
@@ -969,28 +1065,45 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     return fail(ctx, 1, "nstep/dt/nrec differ from the last hr_forward");
   if (ctx->f_history == HR_HISTORY_NONE) return fail(ctx, 2, "the last hr_forward kept no history");
   if (!dis_meas || !g_lambda || !g_mu) return fail(ctx, 1, "null argument");
+  if (nrec > 0 && !rec_eq) return fail(ctx, 1, "null receiver array");
   const bool dev = ctx->flags & HR_DEVICE_PTRS;
+  const int nshot = ctx->f_nshot;
   std::vector<long long> req(nrec);
   if (nrec) {
     if (dev) HIPC(hipMemcpy(req.data(), rec_eq, nrec * sizeof(long long), hipMemcpyDeviceToHost));
     else std::memcpy(req.data(), rec_eq, nrec * sizeof(long long));
   }
   for (int q = 0; q < nrec; ++q)
-    if (ctx->eq_slot[req[q]] != ctx->rec_slot_h[q]) return fail(ctx, 1, "receivers differ from the last hr_forward");
-  const size_t nobs = (size_t)(nstep + 1) * nrec;
+    if (req[q] < 0 || req[q] >= ctx->neq || ctx->eq_slot[req[q]] != ctx->rec_slot_h[q])
+      return fail(ctx, 1, "receivers differ from the last hr_forward");
+  const size_t rows = (size_t)nstep + 1;
+  const size_t nobs = (size_t)nshot * rows * nrec;
   if (ctx->obs_cap < nobs) {
     if (int rc = dalloc(ctx, &ctx->d_obs, nobs)) return rc;
     ctx->obs_cap = nobs;
   }
   HIPC(hipMemcpyAsync(ctx->d_obs, dis_meas, nobs * sizeof(double), dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                       ctx->stream));
-  // adjoint source entries grouped by tile; traces/obs permuted into that order
+  // adjoint source entries: the shared receivers, grouped by tile, repeated per shot;
+  // traces and observations permuted into entry order (device gather)
   SrcBuild sb = build_src(ctx, nrec, req.data());
-  if (int rc = upload(ctx, &ctx->d_rec_tile_ptr, sb.ptr)) return rc;
-  if (int rc = upload(ctx, &ctx->d_rec_node, sb.node)) return rc;
-  if (int rc = upload(ctx, &ctx->d_rec_comp, sb.comp)) return rc;
-  if (int rc = upload(ctx, &ctx->d_rec_m, sb.m)) return rc;
-  // receiver columns of traces / observations in source-entry order (device gather)
+  {
+    const int nt = ntiles(ctx->g);
+    std::vector<int> ptr(1, 0);
+    std::vector<long long> node;
+    std::vector<int> comp;
+    std::vector<double> m;
+    for (int q = 0; q < nshot; ++q) {
+      for (int t = 0; t < nt; ++t) ptr.push_back(q * nrec + sb.ptr[t + 1]);
+      node.insert(node.end(), sb.node.begin(), sb.node.end());
+      comp.insert(comp.end(), sb.comp.begin(), sb.comp.end());
+      m.insert(m.end(), sb.m.begin(), sb.m.end());
+    }
+    if (int rc = upload(ctx, &ctx->d_rec_tile_ptr, ptr)) return rc;
+    if (int rc = upload(ctx, &ctx->d_rec_node, node)) return rc;
+    if (int rc = upload(ctx, &ctx->d_rec_comp, comp)) return rc;
+    if (int rc = upload(ctx, &ctx->d_rec_m, m)) return rc;
+  }
   if (int rc = upload(ctx, &ctx->d_rec_perm, sb.perm)) return rc;
   if (ctx->sorted_cap < nobs) {
     if (int rc = dalloc(ctx, &ctx->d_tr_sorted, nobs)) return rc;
@@ -1002,41 +1115,46 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     std::memset(&h3, 0, sizeof(h3));
     h3.tile_ptr = ctx->d_rec_tile_ptr; h3.node = ctx->d_rec_node; h3.comp = ctx->d_rec_comp;
     h3.m = ctx->d_rec_m; h3.tr = ctx->d_tr_sorted; h3.obs = ctx->d_obs_sorted; h3.nrec = nrec;
+    h3.nstep = nstep; h3.ntiles = ntiles(ctx->g);
     if (int rc = upload(ctx, &ctx->d_src3_adj, std::vector<Src3>{h3})) return rc;
   }
   if (nrec) {
-    HIPC(launch_permute_cols(nstep + 1, nrec, ctx->d_rec_perm, ctx->d_traces, ctx->d_tr_sorted, ctx->stream));
-    HIPC(launch_permute_cols(nstep + 1, nrec, ctx->d_rec_perm, ctx->d_obs, ctx->d_obs_sorted, ctx->stream));
+    HIPC(launch_permute_cols((long long)nshot * rows, nrec, ctx->d_rec_perm, ctx->d_traces, ctx->d_tr_sorted,
+                             ctx->stream));
+    HIPC(launch_permute_cols((long long)nshot * rows, nrec, ctx->d_rec_perm, ctx->d_obs, ctx->d_obs_sorted,
+                             ctx->stream));
   }
-  // gradient accumulators on the RD box
+  // gradient accumulators on the RD box (sum over shots)
   if (int rc = dalloc(ctx, &ctx->d_gl, ctx->nbox)) return rc;
   if (int rc = dalloc(ctx, &ctx->d_gm, ctx->nbox)) return rc;
   HIPC(hipMemsetAsync(ctx->d_gl, 0, ctx->nbox * sizeof(double), ctx->stream));
   HIPC(hipMemsetAsync(ctx->d_gm, 0, ctx->nbox * sizeof(double), ctx->stream));
   const size_t slice = 3 * (size_t)ctx->nbox;
   const int seg = ctx->f_segment;
+  const size_t seg_rows = (size_t)seg + 1;
   if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
-    const size_t need = (size_t)(seg + 1) * slice;
+    const size_t need = (size_t)nshot * seg_rows * slice;
     if (ctx->seg_cap < need) {
       if (int rc = dalloc(ctx, &ctx->d_seg, need)) return rc;
       ctx->seg_cap = need;
     }
   }
   const double sgn[3] = {-1.0, 1.0, -1.0};
-  const size_t nf = (size_t)NCOMP * ctx->g.NN;
+  const size_t nf = field_size(ctx);
+  const size_t nfs = nf * nshot;
   long long recomputed = 0;
   int seg_lo = -1;                       // history slices [seg_lo, seg_lo + seg] live in d_seg
   // adjoint state starts from rest; the forward-state buffers are reused
   if (int rc = zero_state(ctx)) return rc;
-  // keep the adjoint state apart from the recomputation state: use tmp buffers
+  // keep the adjoint state apart from the recomputation state: use extra buffers
   double* adj[3];
   double* adjy[3];
   std::vector<double*> extra;
   if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
     for (int a = 0; a < 6; ++a) {
       double* p = nullptr;
-      if (int rc = dalloc(ctx, &p, nf)) return rc;
-      HIPC(hipMemsetAsync(p, 0, nf * sizeof(double), ctx->stream));
+      if (int rc = dalloc(ctx, &p, nfs)) return rc;
+      HIPC(hipMemsetAsync(p, 0, nfs * sizeof(double), ctx->stream));
       extra.push_back(p);
     }
     for (int a = 0; a < 3; ++a) { adj[a] = extra[a]; adjy[a] = extra[3 + a]; }
@@ -1045,28 +1163,29 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
   }
   int status = 0;
   for (int n = nstep; n >= 1 && !status; --n) {
-    const double* uh = nullptr;
     if (n <= nstep - 1) {
-      if (ctx->f_history == HR_HISTORY_STORE) uh = ctx->d_hist + (size_t)n * slice;
+      const double* uh = nullptr;
+      size_t uh_rows = rows, uh_row = (size_t)n;
+      if (ctx->f_history == HR_HISTORY_STORE) uh = ctx->d_hist;
       else {
         if (seg_lo < 0 || n < seg_lo || n > seg_lo + seg) {
-          // recompute the segment containing n: restore checkpoint, step forward
-          const int c = (n - 1) / seg;       // slices needed: n .. ; segment start c*seg
+          // recompute the segment containing n: restore its checkpoint, step forward
+          const int c = (n - 1) / seg;
           const int s0 = c * seg, s1 = std::min(s0 + seg, nstep);
           double* cp = ctx->ckpt[c];
-          double* sx[3] = {ctx->x[0], ctx->x[1], ctx->x[2]};
-          double* sy[3] = {ctx->y[0], ctx->y[1], ctx->y[2]};
-          for (int a = 0; a < 3; ++a) {
-            ctx->x[a] = sx[a]; ctx->y[a] = sy[a];
-            HIPC(hipMemcpyAsync(ctx->x[a], cp + a * nf, nf * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-          }
-          if (int rc = run_forward_steps(ctx, s0, s1, ctx->d_seg)) { status = rc; break; }
+          for (int a = 0; a < 3; ++a)
+            HIPC(hipMemcpyAsync(ctx->x[a], cp + a * nfs, nfs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+          if (int rc = run_forward_steps(ctx, s0, s1, ctx->d_seg, seg_rows)) { status = rc; break; }
           recomputed += s1 - s0;
           seg_lo = s0;
         }
-        uh = ctx->d_seg + (size_t)(n - seg_lo) * slice;
+        uh = ctx->d_seg;
+        uh_rows = seg_rows;
+        uh_row = (size_t)(n - seg_lo);
       }
-      HIPC(launch_gradient(ctx->g, ctx->box0, ctx->boxn, uh, adj[1], dt, ctx->d_gl, ctx->d_gm, ctx->stream));
+      for (int q = 0; q < nshot; ++q)
+        HIPC(launch_gradient(ctx->g, ctx->box0, ctx->boxn, uh + ((size_t)q * uh_rows + uh_row) * slice, adj[1] + q * nf,
+                             dt, ctx->d_gl, ctx->d_gm, ctx->stream));
     }
     // misfit sources at t_n (stage 1), t_{n-1} (stage 4), mean (stages 2, 3)
     Source s4[4];
@@ -1080,6 +1199,7 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
       if (s == 0) { sp4[s].ra = n; sp4[s].ca = 1.0; }
       else if (s == 3) { sp4[s].ra = n - 1; sp4[s].ca = 1.0; }
       else { sp4[s].ra = n; sp4[s].ca = 0.5; sp4[s].rb = n - 1; sp4[s].cb = 0.5; }
+      // single-shot v1/v2 kernels
       s4[s].tile_ptr = ctx->d_rec_tile_ptr;
       s4[s].node = ctx->d_rec_node;
       s4[s].comp = ctx->d_rec_comp;
@@ -1126,9 +1246,8 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     HIPC(hipMemsetAsync(ctx->tmp2, 0, NN * sizeof(double) * NCOMP, ctx->stream));
   }
   if (misfit) {
-    if (int rc = dalloc(ctx, &ctx->d_part, nstep + 1)) return rc;
     if (!ctx->d_scalar) if (int rc = dalloc(ctx, &ctx->d_scalar, 1)) return rc;
-    HIPC(launch_misfit(nstep, nrec, ctx->d_traces, ctx->d_obs, dt, ctx->d_part, ctx->d_scalar, ctx->stream));
+    if (int rc = misfit_all(ctx, dt, ctx->d_obs, ctx->d_scalar)) return rc;
     HIPC(hipMemcpyAsync(misfit, ctx->d_scalar, sizeof(double), dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                         ctx->stream));
   }

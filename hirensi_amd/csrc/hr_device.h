@@ -78,6 +78,7 @@ hipError_t launch_gradient_finish(const Geo& g, const int* box0, const int* boxn
                                   const double* gmb, double* gl, double* gm, hipStream_t s);
 hipError_t launch_permute_cols(long long nrows, int ncols, const long long* perm, const double* src, double* dst,
                                hipStream_t s);
+hipError_t launch_sum(int n, const double* in, double* out, hipStream_t s);
 hipError_t launch_misfit(int nstep, int nrec, const double* tr, const double* ob, double dt, double* part,
                          double* out, hipStream_t s);
 }  // namespace hr
@@ -102,15 +103,18 @@ constexpr int NT_LAM = 0, NT_MU = 1, NT_MINV_U = 2, NT_MINV_S = 3, NT_RWP = 4, N
 // Small table: [wq (27) | ax bx (NX each) | ay by (NY each) | az bz (NZ each)].
 constexpr int TAB_STRETCH = 27;
 
-struct Src3 {                      // device-resident source entries of one solve, grouped by tile
-  const int* tile_ptr;             // (ntiles + 1)
+struct Src3 {                      // device-resident source entries of one solve, grouped by shot and tile
+  const int* tile_ptr;             // (nshot * ntiles + 1): entries of (shot, tile) at [ptr[s*ntiles+t], ptr[..+1])
   const long long* node;           // lattice node
   const int* comp;                 // component 0..2
   const double* m;                 // M^-1 of that DOF
   const double* val;               // forward: B entries
-  const double* tr;                // adjoint: traces, columns in entry order, rows of nrec
+  const double* ftab;              // forward: time function f(T), f(T+dt/2), f(T+dt): nshot x nstep x 3
+  const double* tr;                // adjoint: traces per shot, (nstep+1) rows of nrec, columns in entry order
   const double* obs;               // adjoint: observations, same layout
-  int nrec;
+  int nrec;                        // adjoint: entries per shot (shared receivers)
+  int nstep;
+  int ntiles;
 };
 
 struct Stage3Args {
@@ -122,9 +126,12 @@ struct Stage3Args {
   const double* tab;               // small table
   const Src3* src;                 // device pointer, used when src_mode != 0
   int set_mask;                    // apply: bit0 G, bit1 K, bit2 C
-  int src_mode;                    // 0 none, 1 forward: ca * B * m, 2 adjoint misfit
+  int src_mode;                    // 0 none, 1 forward: f * B * m, 2 adjoint misfit
+  int step, fcol;                  // forward: f = ftab[(shot * nstep + step) * 3 + fcol]
   int ra, rb;                      // adjoint source rows (rb < 0: none)
-  double ca, cb;
+  double ca, cb;                   // adjoint row weights
+  int nshot;                       // shots in the launch; state vectors are shot-major
+  long long shot_stride;           // elements per shot of a state vector (9 NN)
   double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed)
 };
 

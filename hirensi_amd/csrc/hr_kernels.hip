@@ -1100,6 +1100,15 @@ __global__ void misfit_sum_kernel(int nstep, const double* __restrict__ part, do
   }
 }
 
+// out = in[0] + in[1] + ... + in[n-1], in that order (per-shot misfits)
+__global__ void sum_kernel(int n, const double* __restrict__ in, double* __restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double c = 0.0;
+    for (int q = 0; q < n; ++q) c = c + in[q];
+    out[0] = c;
+  }
+}
+
 // g <- -g for the regular-domain box, scattered to the nodal output
 __global__ void gradient_finish_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int bny, int bnz,
                                        const double* __restrict__ gl_box, const double* __restrict__ gm_box,
@@ -1184,6 +1193,10 @@ hipError_t launch_gradient_finish(const Geo& g, const int* b0, const int* bn, co
 hipError_t launch_permute_cols(long long nrows, int ncols, const long long* perm, const double* src, double* dst,
                                hipStream_t s) {
   hipLaunchKernelGGL(permute_cols_kernel, dim3(grid_for(nrows * ncols)), dim3(256), 0, s, nrows, ncols, perm, src, dst);
+  return hipGetLastError();
+}
+hipError_t launch_sum(int n, const double* in, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, s, n, in, out);
   return hipGetLastError();
 }
 hipError_t launch_misfit(int nstep, int nrec, const double* tr, const double* ob, double dt, double* part, double* out,

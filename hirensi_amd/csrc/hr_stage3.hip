@@ -138,11 +138,22 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 
   const Geo3& g = A.g;
   const int tid = threadIdx.x;
-  // XCD-aware list position: XCD x = blockIdx % 8 takes list entries [x*C, (x+1)*C)
-  const int C = (ntl + 7) / 8;
+  // XCD-aware position in the (shot, tile) list: XCD x = blockIdx % 8 takes entries [x*C, (x+1)*C)
+  const int ntot = ntl * A.nshot;
+  const int C = (ntot + 7) / 8;
   const int li = (blockIdx.x & 7) * C + (blockIdx.x >> 3);
-  if (li >= ntl) return;
-  const int b = tiles[li];
+  if (li >= ntot) return;
+  const int shot = li / ntl;
+  const int b = tiles[li - shot * ntl];
+  // this shot's state vectors (uniform pointers)
+  const long long so = (long long)shot * A.shot_stride;
+  const double* inp[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) inp[q] = A.in[q] ? A.in[q] + so : nullptr;
+  double* const outp = A.out ? A.out + so : nullptr;
+  double* xnp[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) xnp[q] = A.xnew[q] ? A.xnew[q] + so : nullptr;
   const int tx = b % g.tiles0;
   const int ty = (b / g.tiles0) % g.tiles1;
   const int tz = b / (g.tiles0 * g.tiles1);
@@ -207,7 +218,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 #pragma unroll
         for (int q = 0; q < NIN; ++q) {
           const bool need = ALL_U || v2_input<S>(q);
-          XU[c][q] = need ? ldo(A.in[q], off + c * NN8) : 0.0;
+          XU[c][q] = need ? ldo(inp[q], off + c * NN8) : 0.0;
         }
       if constexpr (HAS_RD) {
         lmv[0] = ldo(nodet, off + NT_LAM * NN8);
@@ -223,8 +234,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
           for (int c = 0; c < 3; ++c) {
             double xn0, xn1;
             rk4_update12(XU[c], A.hh, A.h, A.h6, xn0, xn1, c3o[c]);
-            sto(A.xnew[0], off + c * NN8, xn0);
-            sto(A.xnew[1], off + c * NN8, xn1);
+            sto(xnp[0], off + c * NN8, xn0);
+            sto(xnp[1], off + c * NN8, xn1);
           }
         }
       }
@@ -289,7 +300,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 #pragma unroll
             for (int t = 0; t < 3; ++t)
 #pragma unroll
-              for (int q = 0; q < NIN; ++q) XS[t][q] = ldo(A.in[q], off + (3 + 3 * hf + t) * NN8);
+              for (int q = 0; q < NIN; ++q) XS[t][q] = ldo(inp[q], off + (3 + 3 * hf + t) * NN8);
             double s1[3], s2[3], s3[3];
 #pragma unroll
             for (int t = 0; t < 3; ++t) combine<S, NEG>(XS[t], A.hh, A.h, mask, s1[t], s2[t], s3[t]);
@@ -331,8 +342,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
                     rk4_update12(XS[t], A.hh, A.h, A.h6, xn0, xn1, c3v);
                     if constexpr (NS == 9) c3o[3 + m] = c3v;
                     if ((sm >> m) & 1) {
-                      sto(A.xnew[0], off + (3 + m) * NN8, xn0);
-                      sto(A.xnew[1], off + (3 + m) * NN8, xn1);
+                      sto(xnp[0], off + (3 + m) * NN8, xn0);
+                      sto(xnp[1], off + (3 + m) * NN8, xn1);
                     }
                   }
                 }
@@ -544,7 +555,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 #pragma unroll
           for (int m = 0; m < 9; ++m) {
             const bool act = m < 3 ? ua : ((sm >> (m - 3)) & 1);
-            if (act) sto(A.out, off + m * NN8, raw[m]);
+            if (act) sto(outp, off + m * NN8, raw[m]);
           }
         } else {
           const double minv_u = minvu_o, minv_s = minvs_o;
@@ -556,19 +567,22 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
             double src = 0.0;
             if (m < 3 && A.src_mode) {
               const Src3* sp = A.src;
-              const int j0 = sp->tile_ptr[b], j1 = sp->tile_ptr[b + 1];
+              const int tp = shot * sp->ntiles + b;
+              const int j0 = sp->tile_ptr[tp], j1 = sp->tile_ptr[tp + 1];
               for (int jj = j0; jj < j1; ++jj) {
                 if (sp->node[jj] == (long long)n && sp->comp[jj] == m) {
                   const double mm = sp->m[jj];
                   double val;
                   if (A.src_mode == 1) {
-                    val = A.ca * (sp->val[jj] * mm);
+                    const double f = sp->ftab[((size_t)shot * sp->nstep + A.step) * 3 + A.fcol];
+                    val = f * (sp->val[jj] * mm);
                   } else {
                     const size_t nr = (size_t)sp->nrec;
-                    const double va = sp->tr[(size_t)A.ra * nr + jj] - sp->obs[(size_t)A.ra * nr + jj];
+                    const size_t base = (size_t)shot * ((size_t)sp->nstep + 1) * nr + ((size_t)jj - (size_t)shot * nr);
+                    const double va = sp->tr[base + (size_t)A.ra * nr] - sp->obs[base + (size_t)A.ra * nr];
                     val = A.ca * (va * mm);
                     if (A.rb >= 0) {
-                      const double vb = sp->tr[(size_t)A.rb * nr + jj] - sp->obs[(size_t)A.rb * nr + jj];
+                      const double vb = sp->tr[base + (size_t)A.rb * nr] - sp->obs[base + (size_t)A.rb * nr];
                       val = val + A.cb * (vb * mm);
                     }
                   }
@@ -579,10 +593,10 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
             const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
             const double gout = src - minv * raw[m];
             if constexpr (S < 4) {
-              sto(A.out, off + m * NN8, gout);
+              sto(outp, off + m * NN8, gout);
             } else {
               const int cm = (NS == 9) ? m : (m < 3 ? m : 0);
-              sto(A.xnew[2], off + m * NN8, c3o[cm] + A.h6 * gout);
+              sto(xnp[2], off + m * NN8, c3o[cm] + A.h6 * gout);
             }
           }
         }
@@ -594,8 +608,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 
 template <int S, bool TR>
 hipError_t launch3(const Stage3Args& A, int cls, const int* tiles, int ntl, hipStream_t s) {
-  if (ntl <= 0) return hipSuccess;
-  const int nb = 8 * ((ntl + 7) / 8);
+  if (ntl <= 0 || A.nshot <= 0) return hipSuccess;
+  const int nb = 8 * ((ntl * A.nshot + 7) / 8);
   switch (cls) {
     case 0: stage_kernel3<4, 3, true, true, S, TR><<<nb, NT3, 0, s>>>(A, tiles, ntl); break;
     case 1: stage_kernel3<4, 3, true, false, S, TR><<<nb, NT3, 0, s>>>(A, tiles, ntl); break;

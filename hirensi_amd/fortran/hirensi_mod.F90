@@ -113,6 +113,18 @@ module hirensi
       real(c_double), intent(out) :: traces(*)
     end function hr_forward
 
+    integer(c_int) function hr_forward_shots(ctx, nshot, nstep, dt, ftab, nsrc, src_shot, src_eq, src_val, nrec, &
+                                             rec_eq, traces, history, segment) bind(C, name="hr_forward_shots")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nshot, nstep, nsrc, nrec, history, segment
+      real(c_double), value :: dt
+      real(c_double), intent(in) :: ftab(3, *), src_val(*)
+      integer(c_int), intent(in) :: src_shot(*)
+      integer(c_long_long), intent(in) :: src_eq(*), rec_eq(*)
+      real(c_double), intent(out) :: traces(*)
+    end function hr_forward_shots
+
     integer(c_int) function hr_adjoint_gradient(ctx, nstep, dt, nrec, rec_eq, dis_meas, g_lambda, g_mu, misfit) &
         bind(C, name="hr_adjoint_gradient")
       import :: c_int, c_ptr, c_double, c_long_long

@@ -319,3 +319,26 @@ def config2(seed=20261202, with_anomaly=True):
     src = mesh.surface_node(0.0, 0.0)
     return dict(mesh=mesh, model=model, rec_nodes=rec, src_node=src, dt=1.0e-4,
                 nstep=4000, fr=15.0, anomaly_z=zc)
+
+
+def config3(seed=3, nshot=16):
+    """Batched multi-shot gradient: 48x48x24 elements, 6-element PML, the
+    layering of config 2 (true model with the Gaussian -30 % Vs anomaly at
+    z = -12 m, starting model without it), 16 sources on a 4x4 surface grid
+    with Ricker central frequencies drawn from U[10, 20] Hz, 3000 steps of
+    1e-4 s, 225 shared receivers on a 15x15 surface grid spanning the regular
+    domain."""
+    mesh = BoxMesh(48, 48, 24, npml=(6, 6, 6, 6, 6))
+    rng = np.random.default_rng(seed)
+    frs = rng.uniform(10.0, 20.0, nshot)
+    layers = ((150.0, 250.0, 400.0), (1900.0, 2000.0, 2100.0), (-8.0, -18.0))
+    true = layered_model(mesh, *layers, anomaly=(-0.30, 4.0, (0.0, 0.0, -12.0)))
+    start = layered_model(mesh, *layers)
+    half = 0.5 * (mesh.nex - 12) * mesh.h[0]
+    pts = np.round(np.linspace(-half, half, 15) * 2.0) / 2.0
+    rec = np.array([mesh.surface_node(x, y) for y in pts for x in pts], dtype=np.int64)
+    side = int(round(np.sqrt(nshot)))
+    sp = np.round(np.linspace(-0.6 * half, 0.6 * half, side) * 2.0) / 2.0
+    src = np.array([mesh.surface_node(x, y) for y in sp for x in sp], dtype=np.int64)[:nshot]
+    return dict(mesh=mesh, true=true, start=start, rec_nodes=rec, src_nodes=src, frs=frs, dt=1.0e-4,
+                nstep=3000)

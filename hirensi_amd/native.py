@@ -20,6 +20,7 @@ EXPORTS = (
     "hr_set_pml", "hr_set_material", "hr_get_diag_m", "hr_scale", "hr_diagonal_scale",
     "hr_reset_scaling", "hr_apply", "hr_time_function", "hr_forward", "hr_adjoint_gradient",
     "hr_misfit", "hr_recompute_ratio", "hr_set_tiling", "hr_set_profiling", "hr_get_kernel_stats",
+    "hr_forward_shots",
 )
 
 
@@ -66,6 +67,8 @@ def load():
     L.hr_time_function.argtypes = [C.c_double, C.c_int, C.c_double]
     L.hr_time_function.restype = C.c_double
     L.hr_forward.argtypes = [_vp, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int]
+    L.hr_forward_shots.argtypes = [_vp, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp,
+                                   C.c_int, C.c_int]
     L.hr_adjoint_gradient.argtypes = [_vp, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.hr_misfit.argtypes = [_vp, C.c_int, C.c_double, C.c_int, _dp, _dp]
     L.hr_recompute_ratio.argtypes = [_vp]

@@ -9,7 +9,8 @@ Names follow the reference (step_4_3D_Forward_and_Inverse/):
   1323-1352; Program_Main.F90:692-697).
 * :func:`time_function` -- TIME_FUNCTION (IO.F90:541-733).
 * :func:`explicit_rk4_petsc_pml_3d` -- the forward RK4 solve
-  (Solver_PETSc_Explicit.F90:660-1117).
+  (Solver_PETSc_Explicit.F90:660-1117); :func:`explicit_rk4_petsc_pml_3d_shots`
+  advances several shots over one material together.
 * :func:`explicit_rk4_petsc_pml_3d_adjoint_otd` -- adjoint, control pass and
   sign flip (Solver_PETSc_Explicit.F90:1125-1522, Program_Main.F90:400-413).
 * :func:`compute_misfit` -- Compute_Misfit (Solver_PETSc_Inverse.F90:591-650).
@@ -27,6 +28,7 @@ from . import native
 from .native import HrError, HR_K, HR_C, HR_G, HR_HISTORY_NONE, HR_HISTORY_STORE, HR_HISTORY_CHECKPOINT
 
 __all__ = ["Context", "ShellMat", "time_function", "time_table", "explicit_rk4_petsc_pml_3d",
+           "explicit_rk4_petsc_pml_3d_shots",
            "explicit_rk4_petsc_pml_3d_adjoint_otd", "compute_misfit", "HrError",
            "HR_K", "HR_C", "HR_G", "HR_HISTORY_NONE", "HR_HISTORY_STORE", "HR_HISTORY_CHECKPOINT"]
 
@@ -223,6 +225,38 @@ def explicit_rk4_petsc_pml_3d(ctx: Context, nstep, dt, ftab, src_eq, src_val, re
     return traces
 
 
+def explicit_rk4_petsc_pml_3d_shots(ctx: Context, nstep, dt, ftab, src_shot, src_eq, src_val, rec_eq,
+                                    history=HR_HISTORY_STORE, segment=64, traces=None, nshot=None):
+    """Forward solves of several shots over one material, advanced together
+    (``hr_forward_shots``).  ``ftab`` is (nshot, nstep, 3); source entry e
+    belongs to shot ``src_shot[e]``; the receivers are shared.  Returns the
+    traces (nshot, nstep + 1, nrec).  The adjoint and misfit calls that follow
+    act on all shots (observations shaped like the traces)."""
+    nrec = len(rec_eq)
+    if ctx.device_ptrs:
+        if traces is None or nshot is None:
+            raise HrError("device contexts need a traces buffer and nshot")
+        ptrs = (_ptr(ftab), _ptr(src_shot), _ptr(src_eq), _ptr(src_val), _ptr(rec_eq))
+        nsrc = len(src_eq)
+    else:
+        ftab = _f64(ftab)
+        if ftab.ndim != 3 or ftab.shape[1] != nstep or ftab.shape[2] != 3:
+            raise ValueError("time table must be (nshot, nstep, 3)")
+        nshot = ftab.shape[0]
+        src_shot = np.ascontiguousarray(src_shot, dtype=np.int32)
+        src_eq, src_val, rec_eq = _i64(src_eq), _f64(src_val), _i64(rec_eq)
+        nsrc = len(src_eq)
+        if traces is None:
+            traces = np.zeros((nshot, nstep + 1, nrec))
+        ptrs = (ftab.ctypes.data, src_shot.ctypes.data, src_eq.ctypes.data, src_val.ctypes.data,
+                rec_eq.ctypes.data)
+    ctx._check(ctx.L.hr_forward_shots(ctx.h, int(nshot), int(nstep), float(dt), ptrs[0], nsrc,
+                                      ptrs[1] if nsrc else None, ptrs[2] if nsrc else None,
+                                      ptrs[3] if nsrc else None, nrec, ptrs[4] if nrec else None,
+                                      _ptr(traces) if nrec else None, int(history), int(segment)))
+    return traces
+
+
 def explicit_rk4_petsc_pml_3d_adjoint_otd(ctx: Context, nstep, dt, rec_eq, dis_meas, g_lambda=None, g_mu=None,
                                           with_misfit=True, misfit_out=None):
     """OtD adjoint + control pass + sign flip; returns (g_lambda, g_mu, misfit)
@@ -252,7 +286,7 @@ def compute_misfit(ctx: Context, nstep, dt, dis_meas):
     if ctx.device_ptrs:
         raise HrError("compute_misfit returns a host scalar; use a host-pointer context")
     dis_meas = _f64(dis_meas)
-    nrec = dis_meas.shape[1]
+    nrec = dis_meas.shape[-1]
     m = np.zeros(1)
     ctx._check(ctx.L.hr_misfit(ctx.h, int(nstep), float(dt), nrec, dis_meas.ctypes.data, m.ctypes.data))
     return float(m[0])

@@ -105,14 +105,28 @@ int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, 
                const double* src_val, int nrec, const long long* rec_eq, double* traces, int history,
                int segment);
 
+/* Batched shots over one material: nshot forward solves of
+ * EXPLICIT_RK4_PETSC_PML_3D (Solver_PETSc_Explicit.F90:660-1117) advanced
+ * together, one stage-kernel launch per RK4 stage covering every shot.  The
+ * reference runs one shot per solve; this entry point has no single
+ * counterpart there.  ftab[nshot x nstep x 3]; source entry e belongs to shot
+ * src_shot[e]; the receivers are shared; traces[nshot x (nstep+1) x nrec].
+ * hr_adjoint_gradient and hr_misfit then act on all shots of this forward
+ * (observations shot-major, gradient and misfit summed over shots). */
+int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double* ftab, int nsrc,
+                     const int* src_shot, const long long* src_eq, const double* src_val, int nrec,
+                     const long long* rec_eq, double* traces, int history, int segment);
+
 /* OtD adjoint + gradient (Adjoint_OtD, Control_OtD, g <- -g).  Uses the
- * history of the last hr_forward.  dis_meas[(nstep+1) x nrec] measured data.
- * g_lambda, g_mu: nodal (nnode), zero outside the regular-domain node box.
- * misfit (optional, may be NULL): Compute_Misfit of the last forward. */
+ * history of the last hr_forward / hr_forward_shots.  dis_meas[(nstep+1) x
+ * nrec] measured data per shot, shots one after another.
+ * g_lambda, g_mu: nodal (nnode), zero outside the regular-domain node box;
+ * the sum over shots.  misfit (optional, may be NULL): Compute_Misfit of the
+ * last forward, summed over shots. */
 int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long long* rec_eq,
                         const double* dis_meas, double* g_lambda, double* g_mu, double* misfit);
 
-/* Compute_Misfit of the traces of the last hr_forward against dis_meas. */
+/* Compute_Misfit of the traces of the last forward against dis_meas (summed over its shots). */
 int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_meas, double* misfit);
 
 /* Forward steps recomputed per adjoint (1.0 with a stored history). */
