@@ -107,7 +107,7 @@ struct hr_ctx {
   hipStream_t cstream[2] = {nullptr, nullptr};
   hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   int concurrent = 1;
-  // stage-kernel version and tile-class lists (v2)
+  // stage-kernel version and tile-class lists
   int version = DEFAULT_VERSION;
   std::vector<int> zb;                      // z-chunk boundaries (element layers)
   int* d_tl[3] = {nullptr, nullptr, nullptr};
@@ -123,6 +123,14 @@ struct hr_ctx {
 namespace {
 
 size_t field_size(const hr_ctx* ctx) { return (size_t)NCOMP * ctx->g.NN; }
+
+// State-vector slot of component c of lattice node n: displacement block [3 n + c],
+// then stress block [3 NN + 6 n + (c - 3)] (a node's components are contiguous).
+long long slot_of(const Geo& g, int c, long long n) { return c < 3 ? 3 * n + c : 3 * g.NN + 6 * n + (c - 3); }
+void slot_decode(const Geo& g, long long slot, int& c, long long& n) {
+  if (slot < 3 * g.NN) { n = slot / 3; c = (int)(slot - 3 * n); }
+  else { const long long r = slot - 3 * g.NN; n = r / 6; c = 3 + (int)(r - 6 * n); }
+}
 
 int fail(hr_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -218,8 +226,9 @@ SrcBuild build_src(hr_ctx* ctx, int n, const long long* eq) {
   std::vector<std::pair<int, int>> key(n);
   for (int q = 0; q < n; ++q) {
     const long long slot = ctx->eq_slot[eq[q]];
-    const int c = (int)(slot / g.NN);
-    const long long nd = slot - c * g.NN;
+    int c;
+    long long nd;
+    slot_decode(g, slot, c, nd);
     const int i = (int)(nd % g.NX), j = (int)((nd / g.NX) % g.NY), k = (int)(nd / ((long long)g.NX * g.NY));
     key[q] = {tile_of(ctx, i, j, k), q};
   }
@@ -231,8 +240,9 @@ SrcBuild build_src(hr_ctx* ctx, int n, const long long* eq) {
   for (int t = 0; t < nt; ++t) s.ptr[t + 1] += s.ptr[t];
   for (auto& kq : key) {
     const long long slot = ctx->eq_slot[eq[kq.second]];
-    const int c = (int)(slot / g.NN);
-    const long long nd = slot - c * g.NN;
+    int c;
+    long long nd;
+    slot_decode(g, slot, c, nd);
     s.node.push_back(nd);
     s.comp.push_back(c);
     s.m.push_back(ctx->h_minv_u[nd]);
@@ -313,11 +323,7 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
     if (!e0 || !e1) return fail(ctx, 3, "hipEventCreate failed");
     HIPC(hipEventRecord(e0, ctx->stream));
   }
-  if (ctx->version != 3 && A.stage > 0 && ctx->nshot != 1)
-    return fail(ctx, 3, "batched shots need the v3 stage kernel");
-  if (ctx->version == 1) {
-    HIPC(launch_stage(A, ntiles(ctx->g), ctx->stream));
-  } else if (ctx->version == 3) {
+  {
     // 32-bit byte offsets: the node table (6 NN) and the 9 components must stay below 4 GiB
     if ((unsigned long long)ctx->g.NN * 8ull * 9ull >= (1ull << 32))
       return fail(ctx, 3, "lattice of %lld nodes is too large for the v3 stage kernel", ctx->g.NN);
@@ -342,8 +348,6 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
       for (int cls = 0; cls < 3; ++cls)
         HIPC(launch_stage3(B, A.stage, A.transpose, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
     }
-  } else {
-    for (int cls = 0; cls < 3; ++cls) HIPC(launch_stage2(A, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
   }
   if (ctx->profiling) {
     HIPC(hipEventRecord(e1, ctx->stream));
@@ -585,7 +589,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
     for (int j = 0; j < g.NY; ++j)
       for (int i = 0; i < g.NX; ++i)
         for (int c = 0; c < NCOMP; ++c)
-          if (h_active(g, i, j, k, c)) ctx->eq_slot.push_back((long long)c * g.NN + lat(g, i, j, k));
+          if (h_active(g, i, j, k, c)) ctx->eq_slot.push_back(slot_of(g, c, lat(g, i, j, k)));
   ctx->neq = (long long)ctx->eq_slot.size();
   HIPC(hipStreamCreate(&ctx->stream));
   for (int q = 0; q < 2; ++q) {
@@ -665,7 +669,7 @@ int hr_get_store_eq(hr_ctx* ctx, long long* store_eq) {
     for (int bk = 0; bk < ctx->boxn[2]; ++bk)
       for (int bj = 0; bj < ctx->boxn[1]; ++bj)
         for (int bi = 0; bi < ctx->boxn[0]; ++bi)
-          store_eq[t++] = inv[(size_t)c * g.NN + lat(g, ctx->box0[0] + bi, ctx->box0[1] + bj, ctx->box0[2] + bk)];
+          store_eq[t++] = inv[slot_of(g, c, lat(g, ctx->box0[0] + bi, ctx->box0[1] + bj, ctx->box0[2] + bk))];
   return 0;
 }
 
@@ -770,8 +774,9 @@ int hr_get_diag_m(hr_ctx* ctx, double* m) {
   std::vector<double> hm(ctx->neq);
   for (long long e = 0; e < ctx->neq; ++e) {
     const long long slot = ctx->eq_slot[e];
-    const int c = (int)(slot / g.NN);
-    const long long n = slot - c * g.NN;
+    int c;
+    long long n;
+    slot_decode(g, slot, c, n);
     if (c < 3) hm[e] = 1.0 / ctx->h_minv_u[n];
     else hm[e] = (c < 6 ? 1.0 : 2.0) * ctx->h_a[n] * ctx->h_wps[n];
   }
@@ -855,14 +860,12 @@ int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   if (!ctx) return 1;
   const int serial = (version & HR_TILING_SERIAL_CLASSES) ? 1 : 0;
   version &= ~HR_TILING_SERIAL_CLASSES;
-  if (ty < 0 || tz < 0 || version < 0 || version > 3) return fail(ctx, 1, "bad tiling");
+  if (ty < 0 || tz < 0 || (version != 0 && version != 3)) return fail(ctx, 1, "bad tiling (stage kernel version 3)");
   Geo& g = ctx->g;
-  const int v = version ? version : DEFAULT_VERSION;
+  const int v = DEFAULT_VERSION;
   ctx->concurrent = serial ? 0 : 1;
-  const int t = ty ? ty : (v == 1 ? 4 : DEFAULT_TY);
-  if (v == 1 && t != 4) return fail(ctx, 1, "stage kernel v1 tiles are 4 x 4 elements");
-  if (v == 2 && t != 3 && t != 4) return fail(ctx, 1, "stage kernel v2 tiles are 4 x 3 or 4 x 4 elements");
-  if (v == 3 && t != 3) return fail(ctx, 1, "stage kernel v3 tiles are 4 x 3 elements");
+  const int t = ty ? ty : DEFAULT_TY;
+  if (t != 3) return fail(ctx, 1, "stage kernel tiles are 4 x 3 elements");
   ctx->version = v;
   g.tx = 4;
   g.ty = t;
@@ -875,7 +878,6 @@ int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double*
                      int history, int segment) {
   if (int rc = check_ready(ctx)) return rc;
   if (nshot < 1) return fail(ctx, 1, "nshot must be >= 1");
-  if (nshot > 1 && ctx->version != 3) return fail(ctx, 1, "batched shots need the v3 stage kernel");
   if (nstep < 1 || !(dt > 0.0)) return fail(ctx, 1, "nstep must be >= 1 and dt > 0");
   if (!ftab) return fail(ctx, 1, "null time-function table");
   if (nsrc < 0 || nrec < 0) return fail(ctx, 1, "negative counts");
@@ -905,12 +907,12 @@ int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double*
   }
   for (int q = 0; q < nsrc; ++q) {
     if (seq[q] < 0 || seq[q] >= ctx->neq) return fail(ctx, 1, "source equation %lld out of range", seq[q]);
-    if (ctx->eq_slot[seq[q]] / ctx->g.NN >= 3) return fail(ctx, 1, "source equation %lld is not a displacement", seq[q]);
+    if (ctx->eq_slot[seq[q]] >= 3 * ctx->g.NN) return fail(ctx, 1, "source equation %lld is not a displacement", seq[q]);
     if (sshot[q] < 0 || sshot[q] >= nshot) return fail(ctx, 1, "source shot %d out of range", sshot[q]);
   }
   for (int q = 0; q < nrec; ++q) {
     if (req[q] < 0 || req[q] >= ctx->neq) return fail(ctx, 1, "receiver equation %lld out of range", req[q]);
-    if (ctx->eq_slot[req[q]] / ctx->g.NN >= 3) return fail(ctx, 1, "receiver equation %lld is not a displacement", req[q]);
+    if (ctx->eq_slot[req[q]] >= 3 * ctx->g.NN) return fail(ctx, 1, "receiver equation %lld is not a displacement", req[q]);
   }
   const size_t nft = 3 * (size_t)nstep * nshot;
   ctx->ftab.resize(nft);

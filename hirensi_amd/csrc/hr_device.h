@@ -62,9 +62,6 @@ struct StageArgs {
 #include <hip/hip_runtime.h>
 namespace hr {
 // Host launchers (defined with the kernels).
-hipError_t launch_stage(const StageArgs& A, int nblocks, hipStream_t s);
-// v2: tile class 0 = mixed, 1 = regular domain only, 2 = PML only; list of tile ids
-hipError_t launch_stage2(const StageArgs& A, int tile_class, const int* tiles, int ntiles, hipStream_t s);
 hipError_t launch_gather_eq(long long neq, const long long* slot, const double* x, const double* r, double* out,
                             hipStream_t s);
 hipError_t launch_scatter_eq(long long neq, const long long* slot, const double* raw, const double* y, double sc,

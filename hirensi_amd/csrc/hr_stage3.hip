@@ -24,8 +24,11 @@
 //  * each owner thread loads its own node, keeps the diagonal terms (and for
 //    stage 4 x3 and the partial g-sum) in registers, and writes the x1, x2
 //    updates of stage 4 at load time; helpers load the low-side halo;
-//  * 32-bit node offsets on uniform per-component bases; small constant
-//    tables (GLL derivative matrix, weights) in LDS.
+//  * state vectors keep a node's components contiguous (displacement block
+//    [3 n + c], then stress block [3 NN + 6 n + m]), so a window row touches
+//    about half the cache lines of a component-major layout; 32-bit byte
+//    offsets on uniform bases; small constant tables (GLL derivative matrix,
+//    weights) in LDS.
 
 #include <hip/hip_runtime.h>
 #include <type_traits>
@@ -165,7 +168,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   const int ez_first = max(ez0 - 1, 0);
   const int mask = (S == 0) ? A.set_mask : 7;
   const bool useG = mask & 1, useK = mask & 2, useC = mask & 4;
-  const unsigned NN8 = (unsigned)g.NN * 8u;   // component stride in bytes
+  const unsigned NN8 = (unsigned)g.NN * 8u;   // node-table column stride in bytes
+  const unsigned SB = 3u * NN8;                 // byte offset of the stress block of a state vector
   const double* __restrict__ tab = A.tab;
   const double* __restrict__ nodet = A.node;
 
@@ -208,7 +212,9 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
     for (int m = 0; m < 9; ++m) { wv[m] = 0.0; Gv[m] = 0.0; }
     if (in_dom3(g, ii, jj, kk)) {
       const unsigned n = (unsigned)ii + (unsigned)g.NX * ((unsigned)jj + (unsigned)g.NY * (unsigned)kk);
-      const unsigned off = n * 8u;
+      const unsigned off = n * 8u;                 // node table
+      const unsigned offu = n * 24u;               // displacement block of the state vectors
+      const unsigned offs = SB + n * 48u;          // stress block
       const bool ua = u_act3(g, ii, jj, kk);
       const bool pn = HAS_PML && pml3(g, ii, jj, kk);
       // ---- u components: raw inputs
@@ -218,7 +224,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 #pragma unroll
         for (int q = 0; q < NIN; ++q) {
           const bool need = ALL_U || v2_input<S>(q);
-          XU[c][q] = need ? ldo(inp[q], off + c * NN8) : 0.0;
+          XU[c][q] = need ? ldo(inp[q], offu + 8u * c) : 0.0;
         }
       if constexpr (HAS_RD) {
         lmv[0] = ldo(nodet, off + NT_LAM * NN8);
@@ -234,8 +240,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
           for (int c = 0; c < 3; ++c) {
             double xn0, xn1;
             rk4_update12(XU[c], A.hh, A.h, A.h6, xn0, xn1, c3o[c]);
-            sto(xnp[0], off + c * NN8, xn0);
-            sto(xnp[1], off + c * NN8, xn1);
+            sto(xnp[0], offu + 8u * c, xn0);
+            sto(xnp[1], offu + 8u * c, xn1);
           }
         }
       }
@@ -300,7 +306,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 #pragma unroll
             for (int t = 0; t < 3; ++t)
 #pragma unroll
-              for (int q = 0; q < NIN; ++q) XS[t][q] = ldo(inp[q], off + (3 + 3 * hf + t) * NN8);
+              for (int q = 0; q < NIN; ++q) XS[t][q] = ldo(inp[q], offs + 8u * (3 * hf + t));
             double s1[3], s2[3], s3[3];
 #pragma unroll
             for (int t = 0; t < 3; ++t) combine<S, NEG>(XS[t], A.hh, A.h, mask, s1[t], s2[t], s3[t]);
@@ -342,8 +348,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
                     rk4_update12(XS[t], A.hh, A.h, A.h6, xn0, xn1, c3v);
                     if constexpr (NS == 9) c3o[3 + m] = c3v;
                     if ((sm >> m) & 1) {
-                      sto(xnp[0], off + (3 + m) * NN8, xn0);
-                      sto(xnp[1], off + (3 + m) * NN8, xn1);
+                      sto(xnp[0], offs + 8u * m, xn0);
+                      sto(xnp[1], offs + 8u * m, xn1);
                     }
                   }
                 }
@@ -545,7 +551,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
       // ------------------------------------------------------- finish
       if (!(p == 2 && !top_plane) && !halo) {
         const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
-        const unsigned off = n * 8u;
+        const unsigned offu = n * 24u, offs = SB + n * 48u;
+        auto slot = [&](int m) { return m < 3 ? offu + 8u * m : offs + 8u * (m - 3); };
         const bool ua = u_act3(g, i, j, k);
         const int sm = pn ? smask3(g, i, j, k) : 0;
         double raw[9];
@@ -555,7 +562,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 #pragma unroll
           for (int m = 0; m < 9; ++m) {
             const bool act = m < 3 ? ua : ((sm >> (m - 3)) & 1);
-            if (act) sto(outp, off + m * NN8, raw[m]);
+            if (act) sto(outp, slot(m), raw[m]);
           }
         } else {
           const double minv_u = minvu_o, minv_s = minvs_o;
@@ -593,10 +600,10 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
             const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
             const double gout = src - minv * raw[m];
             if constexpr (S < 4) {
-              sto(outp, off + m * NN8, gout);
+              sto(outp, slot(m), gout);
             } else {
               const int cm = (NS == 9) ? m : (m < 3 ? m : 0);
-              sto(xnp[2], off + m * NN8, c3o[cm] + A.h6 * gout);
+              sto(xnp[2], slot(m), c3o[cm] + A.h6 * gout);
             }
           }
         }

@@ -132,12 +132,11 @@ int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_mea
 /* Forward steps recomputed per adjoint (1.0 with a stored history). */
 double hr_recompute_ratio(const hr_ctx* ctx);
 
-/* Stage-kernel configuration: tile of 4 x ty elements in x-y, tz elements
- * per z-chunk, kernel version (1: single-class tiles of 4 x 4; 2: tile
- * classes, ty = 3 or 4; 3: tile classes with per-stage specialisation,
- * ty = 3).  0 keeps a default.  Version 3 runs the tile classes of a stage
- * concurrently on two side streams unless HR_TILING_SERIAL_CLASSES is or-ed
- * into version.  Invalidates the history of the last hr_forward. */
+/* Stage-kernel configuration: tile of 4 x ty elements in x-y (ty = 3), tz
+ * elements per z-chunk, kernel version (3, the only one; 0 keeps it).  The
+ * tile classes of a stage run concurrently on two side streams unless
+ * HR_TILING_SERIAL_CLASSES is or-ed into version.  Invalidates the history of
+ * the last hr_forward. */
 #define HR_TILING_SERIAL_CLASSES 0x100
 int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version);
 
