@@ -100,6 +100,11 @@ struct hr_ctx {
   double *d_obs = nullptr, *d_gl = nullptr, *d_gm = nullptr, *d_part = nullptr, *d_scalar = nullptr;
   double *d_tr_sorted = nullptr, *d_obs_sorted = nullptr;
   size_t obs_cap = 0, sorted_cap = 0;
+  // gradient post-processing: 1/Mg and restriction (geometry only, built once),
+  // regularisation vectors, per-element costs, reduction partials, staging of
+  // host material / gradient vectors
+  double *d_img = nullptr, *d_restr = nullptr, *d_reg = nullptr, *d_cost_el = nullptr, *d_red = nullptr;
+  double *d_pm = nullptr, *d_pg = nullptr;
   double recompute = 1.0;
   hipStream_t stream = nullptr;
   // side streams (default priority) that run tile classes 1 and 2 of a stage
@@ -636,6 +641,7 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   F(ctx->d_rec_tile_ptr); F(ctx->d_rec_node); F(ctx->d_rec_comp); F(ctx->d_rec_m); F(ctx->d_rec_perm);
   F(ctx->d_obs); F(ctx->d_gl); F(ctx->d_gm); F(ctx->d_part); F(ctx->d_scalar); F(ctx->d_tr_sorted);
   F(ctx->d_obs_sorted);
+  F(ctx->d_img); F(ctx->d_restr); F(ctx->d_reg); F(ctx->d_cost_el); F(ctx->d_red); F(ctx->d_pm); F(ctx->d_pg);
   for (int c = 0; c < 3; ++c) F(ctx->d_tl[c]);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto& p : ctx->ev_open) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
@@ -1258,6 +1264,144 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
 }
 
 double hr_recompute_ratio(const hr_ctx* ctx) { return ctx ? ctx->recompute : 0.0; }
+
+}  // extern "C"
+
+namespace {
+
+int ensure_gradient_mass(hr_ctx* ctx) {
+  if (ctx->d_img) return 0;
+  const size_t NN = (size_t)ctx->g.NN;
+  if (int rc = dalloc(ctx, &ctx->d_restr, NN)) return rc;
+  if (int rc = dalloc(ctx, &ctx->d_img, NN)) return rc;
+  HIPC(launch_gradient_mass(ctx->g, ctx->d_img, ctx->d_restr, ctx->stream));
+  return 0;
+}
+
+int check_wreg(hr_ctx* ctx, const double* wreg) {
+  if (wreg && !(wreg[2] != 0.0)) return fail(ctx, 1, "w_Length must be nonzero");
+  return 0;
+}
+
+// Regularisation vectors of lam, mu (device) into ctx->d_reg (lambda, then mu)
+// and the two unscaled costs into ctx->d_red[RED_OUT + {0, 1}].
+constexpr int RED_OUT_COST = 4;
+int regularize(hr_ctx* ctx, int method, const double* wreg, const double* dl, const double* dm) {
+  const size_t NN = (size_t)ctx->g.NN;
+  const long long nbe = reg_box_elements(ctx->g);
+  if (!ctx->d_reg) if (int rc = dalloc(ctx, &ctx->d_reg, 2 * NN)) return rc;
+  if (!ctx->d_cost_el) if (int rc = dalloc(ctx, &ctx->d_cost_el, 2 * (size_t)nbe)) return rc;
+  if (!ctx->d_red) if (int rc = dalloc(ctx, &ctx->d_red, (size_t)dot_partials_size() + 8)) return rc;
+  HIPC(launch_regularization(ctx->g, ctx->d_zc, method == 0, wreg, dl, dm, ctx->d_reg, ctx->d_reg + NN,
+                             ctx->d_cost_el, ctx->stream));
+  const double* xs[2] = {ctx->d_cost_el, ctx->d_cost_el + 1};
+  const double* ys[2] = {nullptr, nullptr};
+  HIPC(launch_dots(2, nbe, 2, xs, ys, ctx->d_red, ctx->d_red + dot_partials_size() + RED_OUT_COST, ctx->stream));
+  return 0;
+}
+
+// device pointers to lam and mu: the caller's, or staged copies of host arrays
+int stage_material(hr_ctx* ctx, const double* lam, const double* mu, const double** dl, const double** dm) {
+  const size_t NN = (size_t)ctx->g.NN;
+  if (ctx->flags & HR_DEVICE_PTRS) {
+    *dl = lam;
+    *dm = mu;
+    return 0;
+  }
+  if (!ctx->d_pm) if (int rc = dalloc(ctx, &ctx->d_pm, 2 * NN)) return rc;
+  if (int rc = to_dev(ctx, ctx->d_pm, lam, NN)) return rc;
+  if (int rc = to_dev(ctx, ctx->d_pm + NN, mu, NN)) return rc;
+  *dl = ctx->d_pm;
+  *dm = ctx->d_pm + NN;
+  return 0;
+}
+
+int copy_scalars(hr_ctx* ctx, double* dst, const double* src, int n) {
+  HIPC(hipMemcpyAsync(dst, src, n * sizeof(double),
+                      (ctx->flags & HR_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hr_gradient_mass(hr_ctx* ctx, double* imgrad, double* restriction) {
+  if (!ctx || !imgrad) return fail(ctx, 1, "null argument");
+  if (int rc = ensure_gradient_mass(ctx)) return rc;
+  const size_t NN = (size_t)ctx->g.NN;
+  if (restriction) if (int rc = from_dev(ctx, restriction, ctx->d_restr, NN)) return rc;
+  if (int rc = from_dev(ctx, imgrad, ctx->d_img, NN)) return rc;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int hr_regularization(hr_ctx* ctx, int method, const double* wreg, const double* lam, const double* mu,
+                      double* reg_lam, double* reg_mu, double* cost) {
+  if (!ctx || !lam || !mu || !reg_lam || !reg_mu) return fail(ctx, 1, "null argument");
+  if (method != HR_REG_TV && method != HR_REG_TN) return fail(ctx, 1, "method must be HR_REG_TV or HR_REG_TN");
+  if (int rc = check_wreg(ctx, wreg)) return rc;
+  const double *dl, *dm;
+  if (int rc = stage_material(ctx, lam, mu, &dl, &dm)) return rc;
+  if (int rc = regularize(ctx, method, wreg, dl, dm)) return rc;
+  const size_t NN = (size_t)ctx->g.NN;
+  if (cost) if (int rc = copy_scalars(ctx, cost, ctx->d_red + dot_partials_size() + RED_OUT_COST, 2)) return rc;
+  if (int rc = from_dev(ctx, reg_lam, ctx->d_reg, NN)) return rc;
+  if (int rc = from_dev(ctx, reg_mu, ctx->d_reg + NN, NN)) return rc;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int hr_regularize_gradient(hr_ctx* ctx, int iter, int method, double factor, const double* wreg, const double* lam,
+                           const double* mu, double* g_lambda, double* g_mu, double* cost_reg, double* reg_factor) {
+  if (!ctx || !lam || !mu || !g_lambda || !g_mu) return fail(ctx, 1, "null argument");
+  if (method != HR_REG_TV && method != HR_REG_TN) return fail(ctx, 1, "method must be HR_REG_TV or HR_REG_TN");
+  if (int rc = check_wreg(ctx, wreg)) return rc;
+  const bool dev = ctx->flags & HR_DEVICE_PTRS;
+  const size_t NN = (size_t)ctx->g.NN;
+  if (int rc = ensure_gradient_mass(ctx)) return rc;
+  const double *dl, *dm;
+  if (int rc = stage_material(ctx, lam, mu, &dl, &dm)) return rc;
+  double *gl = g_lambda, *gm = g_mu;
+  if (!dev) {
+    if (!ctx->d_pg) if (int rc = dalloc(ctx, &ctx->d_pg, 2 * NN)) return rc;
+    gl = ctx->d_pg;
+    gm = ctx->d_pg + NN;
+    if (int rc = to_dev(ctx, gl, g_lambda, NN)) return rc;
+    if (int rc = to_dev(ctx, gm, g_mu, NN)) return rc;
+  }
+  if (int rc = regularize(ctx, method, wreg, dl, dm)) return rc;
+  // ||g||^2 and ||reg||^2 of both fields (VecNorm, Program_Main.F90:433-441)
+  const double* rl = ctx->d_reg;
+  const double* rm = ctx->d_reg + NN;
+  const double* xs[4] = {gl, rl, gm, rm};
+  double* out = ctx->d_red + dot_partials_size();
+  HIPC(launch_dots(4, (long long)NN, 1, xs, xs, ctx->d_red, out, ctx->stream));
+  double h[6];
+  HIPC(hipMemcpyAsync(h, out, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  // adaptive factor; a zero regularisation vector (homogeneous field) gets
+  // factor 0 where the reference divides by zero
+  const double fl = h[1] > 0.0 ? factor * std::sqrt(h[0]) / std::sqrt(h[1]) : 0.0;
+  const double fm = h[3] > 0.0 ? factor * std::sqrt(h[2]) / std::sqrt(h[3]) : 0.0;
+  const int add = iter != 1;
+  HIPC(launch_reg_update((long long)NN, add, fl, fm, rl, rm, ctx->d_img, gl, gm, ctx->stream));
+  if (!dev) {
+    HIPC(hipMemcpyAsync(g_lambda, gl, NN * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipMemcpyAsync(g_mu, gm, NN * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  const double c[2] = {add ? h[4] * fl : 0.0, add ? h[5] * fm : 0.0};
+  const double f2[2] = {fl, fm};
+  if (dev) {
+    if (cost_reg) HIPC(hipMemcpyAsync(cost_reg, c, 2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (reg_factor) HIPC(hipMemcpyAsync(reg_factor, f2, 2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    if (cost_reg) { cost_reg[0] = c[0]; cost_reg[1] = c[1]; }
+    if (reg_factor) { reg_factor[0] = f2[0]; reg_factor[1] = f2[1]; }
+  }
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
 
 int hr_set_profiling(hr_ctx* ctx, int mode) {
   if (!ctx) return 1;

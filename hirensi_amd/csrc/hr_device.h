@@ -78,6 +78,20 @@ hipError_t launch_permute_cols(long long nrows, int ncols, const long long* perm
 hipError_t launch_sum(int n, const double* in, double* out, hipStream_t s);
 hipError_t launch_misfit(int nstep, int nrec, const double* tr, const double* ob, double dt, double* part,
                          double* out, hipStream_t s);
+// gradient post-processing (hr_reg.hip)
+long long reg_box_elements(const Geo& g);
+hipError_t launch_gradient_mass(const Geo& g, double* img, double* restr, hipStream_t s);
+// tv: 1 total variation, 0 TN; wreg: host (w_top, w_bot, w_Length) or nullptr;
+// cost_el: 2 x reg_box_elements(g)
+hipError_t launch_regularization(const Geo& g, const double* zc, int tv, const double* wreg, const double* lam,
+                                 const double* mu, double* rl, double* rm, double* cost_el, hipStream_t s);
+// out[v] = x[v] . y[v] (y[v] == nullptr: sum of x[v]), v < nd <= 4, over n
+// entries spaced by stride; part: dot_partials_size() doubles
+hipError_t launch_dots(int nd, long long n, long long stride, const double* const* x, const double* const* y,
+                       double* part, double* out, hipStream_t s);
+int dot_partials_size();
+hipError_t launch_reg_update(long long n, int add, double fl, double fm, const double* rl, const double* rm,
+                             const double* img, double* gl, double* gm, hipStream_t s);
 }  // namespace hr
 
 namespace hr {

@@ -5,6 +5,9 @@
 !   EXPLICIT_RK4_PETSC_PML_3D           -> hr_forward
 !   *_Adjoint_OtD + *_Control_OtD       -> hr_adjoint_gradient
 !   MatMultAdd / MatMultTransposeAdd    -> hr_apply (or the MatShell adapter)
+!   Assem_R_Mg_3D + iMg inversion       -> hr_gradient_mass
+!   Program_Main.F90:416-481            -> hr_regularize_gradient (hr_regularization)
+!   (wreg: c_loc of (w_top, w_bot, w_Length), or c_null_ptr when unweighted)
 module hirensi
   use iso_c_binding
   implicit none
@@ -12,6 +15,7 @@ module hirensi
   integer(c_int), parameter :: HR_K = 0, HR_C = 1, HR_G = 2
   integer(c_int), parameter :: HR_DEVICE_PTRS = 1
   integer(c_int), parameter :: HR_HISTORY_NONE = 0, HR_HISTORY_STORE = 1, HR_HISTORY_CHECKPOINT = 2
+  integer(c_int), parameter :: HR_REG_TV = 0, HR_REG_TN = 1
 
   type, bind(C) :: hr_mesh_desc
     integer(c_int) :: nex, ney, nez
@@ -145,6 +149,32 @@ module hirensi
       real(c_double), intent(in) :: dis_meas(*)
       real(c_double), intent(out) :: misfit
     end function hr_misfit
+
+    integer(c_int) function hr_gradient_mass(ctx, imgrad, restriction) bind(C, name="hr_gradient_mass")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: imgrad(*), restriction(*)
+    end function hr_gradient_mass
+
+    integer(c_int) function hr_regularization(ctx, method, wreg, lam, mu, reg_lam, reg_mu, cost) &
+        bind(C, name="hr_regularization")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, wreg
+      integer(c_int), value :: method
+      real(c_double), intent(in) :: lam(*), mu(*)
+      real(c_double), intent(out) :: reg_lam(*), reg_mu(*), cost(2)
+    end function hr_regularization
+
+    integer(c_int) function hr_regularize_gradient(ctx, iter, method, factor, wreg, lam, mu, g_lambda, g_mu, &
+                                                   cost_reg, reg_factor) bind(C, name="hr_regularize_gradient")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, wreg
+      integer(c_int), value :: iter, method
+      real(c_double), value :: factor
+      real(c_double), intent(in) :: lam(*), mu(*)
+      real(c_double), intent(inout) :: g_lambda(*), g_mu(*)
+      real(c_double), intent(out) :: cost_reg(2), reg_factor(2)
+    end function hr_regularize_gradient
 
     real(c_double) function hr_recompute_ratio(ctx) bind(C, name="hr_recompute_ratio")
       import :: c_double, c_ptr

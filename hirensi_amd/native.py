@@ -14,13 +14,14 @@ LIB_PATH = os.path.join(HERE, "libhirensi.so")
 HR_DEVICE_PTRS = 1
 HR_K, HR_C, HR_G = 0, 1, 2
 HR_HISTORY_NONE, HR_HISTORY_STORE, HR_HISTORY_CHECKPOINT = 0, 1, 2
+HR_REG_TV, HR_REG_TN = 0, 1
 
 EXPORTS = (
     "hr_ctx_create", "hr_ctx_destroy", "hr_last_error", "hr_get_sizes", "hr_get_store_eq",
     "hr_set_pml", "hr_set_material", "hr_get_diag_m", "hr_scale", "hr_diagonal_scale",
     "hr_reset_scaling", "hr_apply", "hr_time_function", "hr_forward", "hr_adjoint_gradient",
     "hr_misfit", "hr_recompute_ratio", "hr_set_tiling", "hr_set_profiling", "hr_get_kernel_stats",
-    "hr_forward_shots",
+    "hr_forward_shots", "hr_gradient_mass", "hr_regularization", "hr_regularize_gradient",
 )
 
 
@@ -76,6 +77,9 @@ def load():
     L.hr_set_tiling.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.hr_set_profiling.argtypes = [_vp, C.c_int]
     L.hr_get_kernel_stats.argtypes = [_vp, _dp, _dp]
+    L.hr_gradient_mass.argtypes = [_vp, _dp, _dp]
+    L.hr_regularization.argtypes = [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.hr_regularize_gradient.argtypes = [_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     _lib = L

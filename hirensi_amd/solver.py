@@ -14,6 +14,11 @@ Names follow the reference (step_4_3D_Forward_and_Inverse/):
 * :func:`explicit_rk4_petsc_pml_3d_adjoint_otd` -- adjoint, control pass and
   sign flip (Solver_PETSc_Explicit.F90:1125-1522, Program_Main.F90:400-413).
 * :func:`compute_misfit` -- Compute_Misfit (Solver_PETSc_Inverse.F90:591-650).
+* :func:`gradient_mass`, :func:`regularization`, :func:`regularize_gradient` --
+  the gradient post-processing of the inversion loop: Assem_R_Mg_3D and its
+  inversion (Assemble_PETSc_3D.F90:670-748, Control_Problem_Operators.F90:1-24),
+  the TV / TN regularisation (Assemble_PETSc_3D.F90:755-860) and
+  Program_Main.F90:416-481.
 
 Errors raise :class:`HrError` with the library's message, the way the
 reference would stop; there is no CPU fallback.
@@ -25,11 +30,13 @@ import ctypes as C
 import numpy as np
 
 from . import native
-from .native import HrError, HR_K, HR_C, HR_G, HR_HISTORY_NONE, HR_HISTORY_STORE, HR_HISTORY_CHECKPOINT
+from .native import (HrError, HR_K, HR_C, HR_G, HR_HISTORY_NONE, HR_HISTORY_STORE, HR_HISTORY_CHECKPOINT,
+                     HR_REG_TV, HR_REG_TN)
 
 __all__ = ["Context", "ShellMat", "time_function", "time_table", "explicit_rk4_petsc_pml_3d",
            "explicit_rk4_petsc_pml_3d_shots",
-           "explicit_rk4_petsc_pml_3d_adjoint_otd", "compute_misfit", "HrError",
+           "explicit_rk4_petsc_pml_3d_adjoint_otd", "compute_misfit", "gradient_mass", "regularization",
+           "regularize_gradient", "HrError", "HR_REG_TV", "HR_REG_TN",
            "HR_K", "HR_C", "HR_G", "HR_HISTORY_NONE", "HR_HISTORY_STORE", "HR_HISTORY_CHECKPOINT"]
 
 
@@ -290,3 +297,66 @@ def compute_misfit(ctx: Context, nstep, dt, dis_meas):
     m = np.zeros(1)
     ctx._check(ctx.L.hr_misfit(ctx.h, int(nstep), float(dt), nrec, dis_meas.ctypes.data, m.ctypes.data))
     return float(m[0])
+
+
+_REG = {"TV": HR_REG_TV, "TN": HR_REG_TN}
+
+
+def _wreg(wreg):
+    """Weighted_Regularization: None ('No') or (w_top, w_bot, w_Length)."""
+    if wreg is None:
+        return None, None
+    w = _f64(wreg)
+    if w.shape != (3,):
+        raise ValueError("wreg must be (w_top, w_bot, w_Length)")
+    return w, w.ctypes.data
+
+
+def gradient_mass(ctx: Context):
+    """1/Mg of the gradient mass over the regular domain and the 0/1
+    restriction (Assem_R_Mg_3D + Control_Problem_Operators.F90:12-24)."""
+    if ctx.device_ptrs:
+        raise HrError("gradient_mass returns host arrays; use a host-pointer context")
+    img = np.zeros(ctx.nnode)
+    res = np.zeros(ctx.nnode)
+    ctx._check(ctx.L.hr_gradient_mass(ctx.h, img.ctypes.data, res.ctypes.data))
+    return img, res
+
+
+def regularization(ctx: Context, lam, mu, method="TV", wreg=None):
+    """Unscaled regularisation vectors of (lam, mu) and their costs:
+    returns reg_lambda, reg_mu, cost_lambda, cost_mu (Program_Main.F90:419-428)."""
+    if ctx.device_ptrs:
+        raise HrError("regularization returns host arrays; use a host-pointer context")
+    lam, mu = _f64(lam), _f64(mu)
+    w, wp = _wreg(wreg)
+    rl, rm, cost = np.zeros(ctx.nnode), np.zeros(ctx.nnode), np.zeros(2)
+    ctx._check(ctx.L.hr_regularization(ctx.h, _REG[method], wp, lam.ctypes.data, mu.ctypes.data, rl.ctypes.data,
+                                       rm.ctypes.data, cost.ctypes.data))
+    return rl, rm, float(cost[0]), float(cost[1])
+
+
+def regularize_gradient(ctx: Context, it, g_lambda, g_mu, lam, mu, method="TV", factor=0.3, wreg=None):
+    """Sections IV and V.2-3 of the inversion loop (Program_Main.F90:416-481):
+    adds the adaptively scaled regularisation (factor = Regularization_factor_
+    continuation, Adjustments.F90:138) unless ``it == 1`` and applies 1/Mg, in
+    place on g_lambda, g_mu.  Returns (cost_reg_lambda, cost_reg_mu,
+    reg_factor_lambda, reg_factor_mu)."""
+    w, wp = _wreg(wreg)
+    if ctx.device_ptrs:
+        out = np.zeros(4)
+        import torch
+        dev = torch.zeros(4, dtype=torch.float64, device="cuda")
+        ctx._check(ctx.L.hr_regularize_gradient(ctx.h, int(it), _REG[method], float(factor), wp, _ptr(lam), _ptr(mu),
+                                                _ptr(g_lambda), _ptr(g_mu), dev.data_ptr(), dev.data_ptr() + 16))
+        out[:] = dev.cpu().numpy()
+        return tuple(float(v) for v in out)
+    for a in (g_lambda, g_mu):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]):
+            raise ValueError("g_lambda and g_mu must be contiguous float64 arrays (updated in place)")
+    lam, mu = _f64(lam), _f64(mu)
+    cost, fac = np.zeros(2), np.zeros(2)
+    ctx._check(ctx.L.hr_regularize_gradient(ctx.h, int(it), _REG[method], float(factor), wp, lam.ctypes.data,
+                                            mu.ctypes.data, g_lambda.ctypes.data, g_mu.ctypes.data, cost.ctypes.data,
+                                            fac.ctypes.data))
+    return float(cost[0]), float(cost[1]), float(fac[0]), float(fac[1])

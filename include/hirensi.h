@@ -129,6 +129,38 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
 /* Compute_Misfit of the traces of the last forward against dis_meas (summed over its shots). */
 int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_meas, double* misfit);
 
+/* Gradient post-processing on the regular domain (Program_Main.F90:416-484).
+ * Nodal vectors have nnode entries; only regular-domain elements contribute
+ * (MTYPE == 1).  wreg: NULL for Weighted_Regularization = 'No', else the host
+ * triple (w_top, w_bot, w_Length) of the weighted scheme
+ * (Adjustments.F90:174-177).  Scalar outputs follow HR_DEVICE_PTRS like arrays. */
+#define HR_REG_TV 0   /* Assem_TV_TN_Regularization (Assemble_PETSc_3D.F90:755-860) */
+#define HR_REG_TN 1   /* MatMult(Reg_PETSC, .) with Reg from Assem_R_Mg_3D */
+
+/* 1/Mg of the gradient mass (Assem_R_Mg_3D, Assemble_PETSc_3D.F90:670-748)
+ * as inverted by Control_Problem_Operators.F90:12-24: imgrad = 1/Mg where
+ * Mg > 1e-10, else 0; restriction (may be NULL) = 1 there, else 0. */
+int hr_gradient_mass(hr_ctx* ctx, double* imgrad, double* restriction);
+
+/* Unscaled regularisation vectors of lam and mu (nodal) and their costs
+ * cost[2] = (lambda, mu), may be NULL.  HR_REG_TV: ELEMENT_TV_TN_Reg_3D
+ * (Element_Matrices_3D.F90:1899-2075) summed over the regular domain, the
+ * cost with the reference's integer-truncated numerator; HR_REG_TN: R m with R
+ * of ELEMENT_R_Mg_3D (Element_Matrices_3D.F90:1555-1718), cost 1/2 m.R m
+ * (Program_Main.F90:419-428,464-474 before scaling). */
+int hr_regularization(hr_ctx* ctx, int method, const double* wreg, const double* lam, const double* mu,
+                      double* reg_lam, double* reg_mu, double* cost);
+
+/* Sections IV and V.2-3 of the inversion loop (Program_Main.F90:416-481) on
+ * the misfit gradient g_lambda, g_mu (in place, nodal, as returned by
+ * hr_adjoint_gradient): regularisation vectors of (lam, mu), adaptive factor
+ * f = factor * ||g|| / ||reg|| per field (0 when ||reg|| = 0, where the
+ * reference divides by zero), g <- g + f reg unless iter == 1, g <- g / Mg.
+ * cost_reg[2] (may be NULL): f * cost per field, 0 when iter == 1;
+ * reg_factor[2] (may be NULL): the two factors. */
+int hr_regularize_gradient(hr_ctx* ctx, int iter, int method, double factor, const double* wreg, const double* lam,
+                           const double* mu, double* g_lambda, double* g_mu, double* cost_reg, double* reg_factor);
+
 /* Forward steps recomputed per adjoint (1.0 with a stored history). */
 double hr_recompute_ratio(const hr_ctx* ctx);
 

@@ -538,6 +538,85 @@ void element_g(const double* XT, const double* EU, const double* EP, double* g_m
       }
 }
 
+// --------------------------------------------------------- ELEMENT_R_Mg_3D
+// Element_Matrices_3D.F90:1555-1718: ER = sum_q (H1 + H2 + H3) * FAC0 * Fac1 with
+// Hk(I,J) = DFX(I,k) DFX(J,k), and the diagonal of EM_g = sum_q FN FN^T FAC0.
+// wreg = {w_top, w_bot, w_Length} switches the weighted scheme on (Fac1 = weight), NULL off.
+double reg_weight(const QPoint& q, const double* XT, const double* wreg) {
+  if (!wreg) return 1.0;
+  double Z = 0.0;
+  for (int n = 0; n < 27; ++n) Z += q.FN[n] * XT[2 + 3 * n];
+  const double f1 = 1.0 - std::fabs(Z / wreg[2]);
+  const double f2 = std::fabs(Z / wreg[2]);
+  return f1 * wreg[0] + f2 * wreg[1];
+}
+
+// Fortran REAL -> default INTEGER assignment as flang compiles it: truncation
+// toward zero, saturating at the int32 range, NaN -> 0.
+double f90_int4(double x) {
+  if (x != x) return 0.0;
+  const double t = std::trunc(x);
+  return t < -2147483648.0 ? -2147483648.0 : (t > 2147483647.0 ? 2147483647.0 : t);
+}
+
+void element_r_mg(const double* XT, const double* wreg, double* ER, double* EMg) {
+  std::vector<double> EM(27 * 27, 0.0);
+  for (int n = 0; n < 27 * 27; ++n) ER[n] = 0.0;
+  QPoint q;
+  for (int lz = 0; lz < 3; ++lz)
+    for (int ly = 0; ly < 3; ++ly)
+      for (int lx = 0; lx < 3; ++lx) {
+        qpoint(XT, lx, ly, lz, q, true);
+        const double fac1 = reg_weight(q, XT, wreg);
+        const double* D = q.DFX;
+        for (int J = 0; J < 27; ++J)
+          for (int I = 0; I < 27; ++I) {
+            const double h = (D[I] * D[J] + D[I + 27] * D[J + 27]) + D[I + 54] * D[J + 54];
+            ER[I + 27 * J] = ER[I + 27 * J] + h * q.fac0 * fac1;
+            EM[I + 27 * J] = EM[I + 27 * J] + q.FN[I] * q.FN[J] * q.fac0;
+          }
+      }
+  for (int I = 0; I < 27; ++I) EMg[I] = EM[I + 27 * I];
+}
+
+// ---------------------------------------------------- ELEMENT_TV_TN_Reg_3D
+// Element_Matrices_3D.F90:1899-2075 (total variation, epsilon_TV = 0.01):
+// per point vec = ER_q m, den = sqrt(m . vec + eps), Reg += vec / den * FAC0 * Fac1,
+// cost += (m . vec) / den * FAC0 * Fac1; cost[0] lambda, cost[1] mu.
+void element_tv(const double* XT, const double* EMu, const double* ELam, const double* wreg, double* RegMu,
+                double* RegLam, double* cost) {
+  const double eps = 0.01;
+  for (int n = 0; n < 27; ++n) { RegMu[n] = 0.0; RegLam[n] = 0.0; }
+  cost[0] = 0.0; cost[1] = 0.0;
+  QPoint q;
+  double ER[27 * 27], vh[27];
+  for (int lz = 0; lz < 3; ++lz)
+    for (int ly = 0; ly < 3; ++ly)
+      for (int lx = 0; lx < 3; ++lx) {
+        qpoint(XT, lx, ly, lz, q, true);
+        const double fac1 = reg_weight(q, XT, wreg);
+        const double* D = q.DFX;
+        for (int J = 0; J < 27; ++J)
+          for (int I = 0; I < 27; ++I) ER[I + 27 * J] = (D[I] * D[J] + D[I + 27] * D[J + 27]) + D[I + 54] * D[J + 54];
+        for (int f = 0; f < 2; ++f) {
+          const double* m = f == 0 ? EMu : ELam;
+          double* reg = f == 0 ? RegMu : RegLam;
+          for (int I = 0; I < 27; ++I) {
+            double s = 0.0;
+            for (int J = 0; J < 27; ++J) s += ER[I + 27 * J] * m[J];
+            vh[I] = s;
+          }
+          const double num = dot27(m, vh);
+          const double den = std::sqrt(num + eps);
+          for (int I = 0; I < 27; ++I) reg[I] = reg[I] + (vh[I] / den) * q.fac0 * fac1;
+          // the reference's `numerator` is implicitly INTEGER (IMPLICIT DOUBLE
+          // PRECISION (A-H,O-Z)): the cost uses the dot product converted to a
+          // saturating, truncating 32-bit integer
+          cost[1 - f] = cost[1 - f] + (f90_int4(num) / den) * q.fac0 * fac1;
+        }
+      }
+}
+
 // ------------------------------------------------------------ TIME_FUNCTION
 double time_function(double T, int lcase, double fr) {
   const double pi = 3.14159265358979323846264338327950288419716939937510;
@@ -820,6 +899,67 @@ void or_element_g(const double* xt, const double* eu, const double* ep, double* 
   element_g(xt, eu, ep, g_mu, g_lam);
 }
 double or_time_function(double t, int lcase, double fr) { return time_function(t, lcase, fr); }
+void or_element_r_mg(const double* xt, const double* wreg, double* er, double* emg) { element_r_mg(xt, wreg, er, emg); }
+void or_element_tv(const double* xt, const double* e_mu, const double* e_lam, const double* wreg, double* reg_mu,
+                   double* reg_lam, double* cost) {
+  element_tv(xt, e_mu, e_lam, wreg, reg_mu, reg_lam, cost);
+}
+
+// Assem_R_Mg_3D (Assemble_PETSc_3D.F90:670-748) + inversion (Control_Problem_Operators.F90:12-24):
+// regular-domain elements only; imgrad = 1/Mg where Mg > 1e-10 else 0, restriction 1/0;
+// reg_tn (optional): the assembled ER applied to lam and mu (MatMult(Reg_PETSC, .)).
+void or_gradient_mass(void* s, const double* wreg, double* imgrad, double* restriction) {
+  System* S = (System*)s;
+  std::vector<double> mg(S->nnode, 0.0);
+  for (i64 e = 0; e < S->nel; ++e) {
+    if (S->mtel[e] != 1) continue;
+    double XT[81], ER[729], EMg[27];
+    element_xt(*S, e, XT, nullptr, nullptr);
+    element_r_mg(XT, wreg, ER, EMg);
+    for (int n = 0; n < 27; ++n) mg[S->inod[e * 27 + n]] += EMg[n];
+  }
+  for (i64 k = 0; k < S->nnode; ++k) {
+    if (mg[k] > 1.0e-10) { imgrad[k] = 1.0 / mg[k]; restriction[k] = 1.0; }
+    else { imgrad[k] = 0.0; restriction[k] = 0.0; }
+  }
+}
+
+// Regularisation vectors over the regular-domain elements.  method 0: total variation
+// (Assem_TV_TN_Regularization, Assemble_PETSc_3D.F90:755-860), cost from the element
+// routine; method 1: TN, the ER of Assem_R_Mg_3D applied to lam and mu, cost 0.5 reg . m
+// (Program_Main.F90:426-428,461-464).  cost[0] lambda, cost[1] mu.
+void or_regularization(void* s, int method, const double* wreg, const double* lam, const double* mu, double* reg_lam,
+                       double* reg_mu, double* cost) {
+  System* S = (System*)s;
+  for (i64 k = 0; k < S->nnode; ++k) { reg_lam[k] = 0.0; reg_mu[k] = 0.0; }
+  cost[0] = 0.0; cost[1] = 0.0;
+  for (i64 e = 0; e < S->nel; ++e) {
+    if (S->mtel[e] != 1) continue;
+    double XT[81], el[27], em[27], rl[27], rm[27], c[2];
+    element_xt(*S, e, XT, nullptr, nullptr);
+    for (int n = 0; n < 27; ++n) { el[n] = lam[S->inod[e * 27 + n]]; em[n] = mu[S->inod[e * 27 + n]]; }
+    if (method == 0) {
+      element_tv(XT, em, el, wreg, rm, rl, c);
+      cost[0] = cost[0] + c[0];
+      cost[1] = cost[1] + c[1];
+    } else {
+      double ER[729], EMg[27];
+      element_r_mg(XT, wreg, ER, EMg);
+      for (int I = 0; I < 27; ++I) {
+        double a = 0.0, b = 0.0;
+        for (int J = 0; J < 27; ++J) { a += ER[I + 27 * J] * el[J]; b += ER[I + 27 * J] * em[J]; }
+        rl[I] = a; rm[I] = b;
+      }
+    }
+    for (int n = 0; n < 27; ++n) { reg_lam[S->inod[e * 27 + n]] += rl[n]; reg_mu[S->inod[e * 27 + n]] += rm[n]; }
+  }
+  if (method != 0) {
+    double a = 0.0, b = 0.0;
+    for (i64 k = 0; k < S->nnode; ++k) { a += reg_lam[k] * lam[k]; b += reg_mu[k] * mu[k]; }
+    cost[0] = 0.5 * a;
+    cost[1] = 0.5 * b;
+  }
+}
 
 void or_set_threads(int n) {
 #ifdef _OPENMP

@@ -58,6 +58,10 @@ def lib():
         L.or_control.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int64, _ip, _dp, _dp, _dp, _dp]
         L.or_misfit.argtypes = [C.c_int, C.c_double, C.c_int, _ip, _dp, C.c_int64, _dp]
         L.or_misfit.restype = C.c_double
+        L.or_element_r_mg.argtypes = [_dp, C.c_void_p, _dp, _dp]
+        L.or_element_tv.argtypes = [_dp, _dp, _dp, C.c_void_p, _dp, _dp, _dp]
+        L.or_gradient_mass.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp]
+        L.or_regularization.argtypes = [C.c_void_p, C.c_int, C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -81,6 +85,8 @@ def ref():
         R.ref_element_ea.argtypes = [_dp, _dp, _dp, _dp, C.c_int, _dp]
         R.ref_element_fs.argtypes = [_dp, C.c_int, _dp]
         R.ref_element_g.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        R.ref_element_r_mg.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
+        R.ref_element_tv.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp]
         _ref = R
     return _ref
 
@@ -157,6 +163,42 @@ class _Elem:
         return gm, gl
 
 
+    # Regularisation weighting: wreg = (w_top, w_bot, w_Length) or None for the
+    # unweighted scheme (Weighted_Regularization = 'No', Adjustments.F90:174-177).
+    def element_r_mg(self, xt, wreg=None):
+        er = np.zeros(27 * 27)
+        emg = np.zeros(27)
+        if self.p == "ref":
+            w = np.zeros(3) if wreg is None else np.ascontiguousarray(wreg, dtype=np.float64)
+            self._fn("element_r_mg")(self._xt(xt), int(wreg is not None), w, er, emg)
+        else:
+            w = _opt(wreg)
+            self._fn("element_r_mg")(self._xt(xt), _addr(w), er, emg)
+        return er.reshape(27, 27).T, emg
+
+    def element_tv(self, xt, e_mu, e_lam, wreg=None):
+        rm = np.zeros(27)
+        rl = np.zeros(27)
+        cost = np.zeros(2)
+        em = np.ascontiguousarray(e_mu, dtype=np.float64)
+        el = np.ascontiguousarray(e_lam, dtype=np.float64)
+        if self.p == "ref":
+            w = np.zeros(3) if wreg is None else np.ascontiguousarray(wreg, dtype=np.float64)
+            self._fn("element_tv")(self._xt(xt), em, el, int(wreg is not None), w, rm, rl, cost)
+        else:
+            w = _opt(wreg)
+            self._fn("element_tv")(self._xt(xt), em, el, _addr(w), rm, rl, cost)
+        return rm, rl, cost           # cost = (lambda part, mu part)
+
+
+def _opt(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _addr(a):
+    return None if a is None else a.ctypes.data
+
+
 def oracle_elem():
     return _Elem(lib(), "or")
 
@@ -215,6 +257,26 @@ class System:
         m = np.zeros(self.neq)
         lib().or_get_diag_m(self.h, m)
         return m
+
+    def gradient_mass(self, wreg=None):
+        """Nodal 1/Mg over regular-domain elements (0 where Mg <= 1e-10) and
+        the 0/1 restriction mask."""
+        img = np.zeros(self.mesh.nnode)
+        res = np.zeros(self.mesh.nnode)
+        w = _opt(wreg)
+        lib().or_gradient_mass(self.h, _addr(w), img, res)
+        return img, res
+
+    def regularization(self, method, lam, mu, wreg=None):
+        """Unscaled nodal regularisation vectors and costs.  method 'TV' or 'TN'.
+        Returns reg_lam, reg_mu, cost_lam, cost_mu."""
+        n = self.mesh.nnode
+        rl, rm, cost = np.zeros(n), np.zeros(n), np.zeros(2)
+        w = _opt(wreg)
+        lib().or_regularization(self.h, {"TV": 0, "TN": 1}[method], _addr(w),
+                                np.ascontiguousarray(lam, dtype=np.float64),
+                                np.ascontiguousarray(mu, dtype=np.float64), rl, rm, cost)
+        return rl, rm, cost[0], cost[1]
 
     def matmult_add(self, which, x, y, transpose=False):
         z = np.zeros(self.neq)

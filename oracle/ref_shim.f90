@@ -8,7 +8,7 @@ module hr_ref_shim
   use PARAMETERS
   implicit none
   external :: ELEMENT_K_3D, ELEMENT_M_3D, ELEMENT_EN_3D, ELEMENT_EA_3D, ELEMENT_FS_3D
-  external :: ELEMENT_g_3D, SHAPE_3_27, PML_FINDER_3D
+  external :: ELEMENT_g_3D, SHAPE_3_27, PML_FINDER_3D, ELEMENT_R_Mg_3D, ELEMENT_TV_TN_Reg_3D
 contains
 
   subroutine set_globals()
@@ -118,6 +118,44 @@ contains
     integer :: inod(27, 1), ngp(1), mtel(1)
     call one_element(xt, xyz, inod, ngp, mtel, pmat, 1, 1.0d0)
     call ELEMENT_g_3D(xyz, inod, ngp, 1, eu, ep, g_mu, g_lambda)
+  end subroutine
+
+  ! weighted regularisation: wreg = (w_top, w_bot, w_Length) when weighted /= 0
+  subroutine set_weighting(weighted, wreg)
+    integer(c_int), value :: weighted
+    real(c_double), intent(in) :: wreg(3)
+    if (weighted /= 0) then
+      Weighted_Regularization = 'Yes'
+      w_top = wreg(1)
+      w_bot = wreg(2)
+      w_Length = wreg(3)
+    else
+      Weighted_Regularization = 'No '
+    end if
+  end subroutine
+
+  subroutine ref_element_r_mg(xt, weighted, wreg, er, emg) bind(C, name="ref_element_r_mg")
+    real(c_double), intent(in) :: xt(3, 27), wreg(3)
+    integer(c_int), value :: weighted
+    real(c_double), intent(out) :: er(27, 27), emg(27)
+    real(8) :: xyz(27, 3), pmat(2, 8)
+    integer :: inod(27, 1), ngp(1), mtel(1)
+    call one_element(xt, xyz, inod, ngp, mtel, pmat, 1, 1.0d0)
+    call set_weighting(weighted, wreg)
+    call ELEMENT_R_Mg_3D(xyz, inod, ngp, mtel, 1, er, emg)
+  end subroutine
+
+  subroutine ref_element_tv(xt, e_mu, e_lambda, weighted, wreg, reg_mu, reg_lambda, cost) bind(C, name="ref_element_tv")
+    real(c_double), intent(in) :: xt(3, 27), e_mu(27), e_lambda(27), wreg(3)
+    integer(c_int), value :: weighted
+    real(c_double), intent(out) :: reg_mu(27), reg_lambda(27), cost(2)
+    real(8) :: xyz(27, 3), pmat(2, 8), c_lam, c_mu
+    integer :: inod(27, 1), ngp(1), mtel(1)
+    call one_element(xt, xyz, inod, ngp, mtel, pmat, 1, 1.0d0)
+    call set_weighting(weighted, wreg)
+    call ELEMENT_TV_TN_Reg_3D(xyz, inod, ngp, mtel, 1, e_mu, e_lambda, reg_mu, reg_lambda, c_lam, c_mu)
+    cost(1) = c_lam
+    cost(2) = c_mu
   end subroutine
 
 end module hr_ref_shim

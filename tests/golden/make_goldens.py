@@ -1,5 +1,6 @@
 """Writes tests/golden/elements.npz: inputs and outputs of the reference's own
-element routines (SHAPE_3_27, PML_FINDER_3D, ELEMENT_K/M/EN/EA/FS/g_3D),
+element routines (SHAPE_3_27, PML_FINDER_3D, ELEMENT_K/M/EN/EA/FS/g_3D,
+ELEMENT_R_Mg_3D, ELEMENT_TV_TN_Reg_3D),
 compiled from its Fortran sources into oracle/_ref/libhr_ref.so by
 oracle/ref.mk.  Run in the build container:
 
@@ -18,6 +19,8 @@ sys.path.insert(0, ROOT)
 from oracle import oracle as O           # noqa: E402
 from hirensi_amd.mesh import BoxMesh, LOCAL_NODES  # noqa: E402
 
+# (w_top, w_bot, w_Length) of Adjustments.F90:174-176
+WREG = np.array([50.0, 1.0, 45.0])
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "elements.npz")
 
 
@@ -63,6 +66,14 @@ def main():
         gm, gl = r.element_g(xt, eu, ep)
         data[f"{name}/g_mu"] = gm
         data[f"{name}/g_lambda"] = gl
+        for tag, w in (("", None), ("_w", WREG)):
+            er, emg = r.element_r_mg(xt, w)
+            data[f"{name}/ER{tag}"] = er
+            data[f"{name}/EMg{tag}"] = emg
+            rm, rl, cost = r.element_tv(xt, mu, lam, w)
+            data[f"{name}/TV_mu{tag}"] = rm
+            data[f"{name}/TV_lambda{tag}"] = rl
+            data[f"{name}/TV_cost{tag}"] = cost
     top = element_xt((0.5, 0.5, -0.5))
     data["fs/xt"] = top
     data["fs/face5"] = r.element_fs(top, 5)
