@@ -45,12 +45,26 @@ _vp = C.c_void_p
 _dp = C.c_void_p      # double* passed as raw address (host numpy or device pointer)
 
 
+def _init_torch_runtime():
+    """torch ships its own HIP runtime.  Device-pointer contexts share HBM
+    with torch tensors, so the library must bind to the runtime torch uses:
+    that runtime is loaded and initialised before the library when a GPU is
+    present (loading the library first leaves torch without a device)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def load():
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise HrError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    _init_torch_runtime()
     L = C.CDLL(LIB_PATH)
     L.hr_ctx_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(_vp)]
     L.hr_ctx_destroy.argtypes = [_vp]
