@@ -126,7 +126,7 @@ template <int TX, int TY, bool HAS_RD, bool HAS_PML, int S, bool TR>
 __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S <= 3 ? 3 : 2)) stage_kernel3(const Stage3Args A, const int* __restrict__ tiles, int ntl) {
   using SH = Shape3<TX, TY>;
   constexpr int WX = SH::WX, WP = SH::WP, NEW = SH::NEW, OX = SH::OX, OC = SH::OC, HP = SH::HP;
-  static_assert(3 * OC <= NT3, "three owner planes must fit the workgroup");
+  static_assert(OC <= 64 && NT3 == 256, "one owner plane group per wavefront, plus one helper wavefront");
   constexpr bool NEG = (S > 0) && TR;
   constexpr int NS = (S == 4) ? (HAS_PML ? 9 : 3) : 1;   // stage-4 register carry width
 
@@ -180,9 +180,11 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
                         : (c == 1 ? (a == 0 ? 2.0 : (a == 1 ? 0.0 : -2.0)) : (a == 0 ? -0.5 : (a == 1 ? 0.5 : 1.5)));
   }
 
-  // owner mapping: group P owns the lattice planes with k % 3 == P
-  const bool owner = tid < 3 * OC;
-  const int P = tid / OC, oc = tid - P * OC;
+  // owner mapping: wavefront P < 3 is the group that owns the lattice planes with
+  // k % 3 == P (lanes oc < OC); wavefront 3 loads halo positions.  Keeping a
+  // group inside one wavefront means that a wavefront runs one load path per slab.
+  const int P = tid >> 6, oc = tid & 63;
+  const bool owner = P < 3 && oc < OC;
   const int ix = oc % OX, iy = oc / OX;
   const int i = 2 * ex0 + ix, j = 2 * ey0 + iy;
   const bool owned = owner && i < own_i1 && j < own_j1;
@@ -386,11 +388,15 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
     {
       const int nnew = 2 * ez + 3 - kfirst;
       int hr_, nh;
-      if (first) { hr_ = tid; nh = NT3; }
-      else {
+      if (first) {
+        // all three groups load a plane: wavefront 3 and the spare lanes take the halo
+        hr_ = P == 3 ? oc : (oc >= OC ? 64 + P * (64 - OC) + (oc - OC) : -1);
+        nh = 64 + 3 * (64 - OC);
+      } else {
+        // the group whose plane is kept from the last slab helps wavefront 3
         const int idleP = (2 * ez) % 3;
-        hr_ = tid >= 3 * OC ? tid - 3 * OC : (P == idleP ? (NT3 - 3 * OC) + oc : -1);
-        nh = NT3 - 2 * OC;
+        hr_ = P == 3 ? oc : (P == idleP ? 64 + oc : -1);
+        nh = 128;
       }
       if (hr_ >= 0) {
         for (int hx = hr_; hx < nnew * HP; hx += nh) {
