@@ -129,15 +129,21 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   static_assert(OC <= 64 && NT3 == 256, "one owner plane group per wavefront, plus one helper wavefront");
   constexpr bool NEG = (S > 0) && TR;
   constexpr int NS = (S == 4) ? (HAS_PML ? 9 : 3) : 1;   // stage-4 register carry width
+  // LDS ring of lattice planes.  With five slots the two planes that the next
+  // slab loads never hold a plane of the current slab, so a wavefront that has
+  // finished the current slab can start loading the next one without waiting
+  // for the others (no end-of-slab barrier).  Only the PML class uses five: the
+  // mixed class would not fit twice per CU, and the regular-domain class runs
+  // faster at four blocks per CU with three slots than at three with five.
+  constexpr int RING = (HAS_PML && !HAS_RD) ? 5 : 3;
 
-  __shared__ double sV[HAS_RD ? 3 : 1][HAS_RD ? 3 : 1][HAS_RD ? WP : 1];     // K input of u (v2)
-  __shared__ double sW[HAS_PML ? 3 : 1][HAS_PML ? 9 : 1][HAS_PML ? WP : 1];  // F-weighted u inputs, 3*d + i
-  __shared__ double sG[HAS_PML ? 3 : 1][HAS_PML ? 9 : 1][HAS_PML ? WP : 1];  // nodal PML flux, 3*i + k
-  __shared__ double sLM[HAS_RD ? 3 : 1][2][HAS_RD ? WP : 1];                  // lam, mu of the window nodes
+  __shared__ double sV[HAS_RD ? RING : 1][HAS_RD ? 3 : 1][HAS_RD ? WP : 1];     // K input of u (v2)
+  __shared__ double sW[HAS_PML ? RING : 1][HAS_PML ? 9 : 1][HAS_PML ? WP : 1];  // F-weighted u inputs, 3*d + i
+  __shared__ double sG[HAS_PML ? RING : 1][HAS_PML ? 9 : 1][HAS_PML ? WP : 1];  // nodal PML flux, 3*i + k
+  __shared__ double sLM[HAS_RD ? RING : 1][2][HAS_RD ? WP : 1];                  // lam, mu of the window nodes
   __shared__ double sSig[HAS_RD ? NEW : 1][6][27];                            // regular domain W * sigma
   __shared__ double sWq[27];
   __shared__ double sDM[9];
-  __shared__ int sType[NEW];
 
   const Geo3& g = A.g;
   const int tid = threadIdx.x;
@@ -382,8 +388,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
     // ------------------------------------------------------------ loads
     if (owner && k >= kfirst) {
       const bool own_plane = k >= 2 * ez0 && (k < 2 * ez_end || k == g.NZ - 1);
-      if (owned) load(std::true_type{}, i, j, k, rown, k % 3, own_plane);
-      else load(std::false_type{}, i, j, k, rown, k % 3, false);
+      if (owned) load(std::true_type{}, i, j, k, rown, k % RING, own_plane);
+      else load(std::false_type{}, i, j, k, rown, k % RING, false);
     }
     {
       const int nnew = 2 * ez + 3 - kfirst;
@@ -405,15 +411,9 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
           if (q < 2 * WX) { ly_ = q / WX; lx_ = q - ly_ * WX; }
           else { const int q2 = q - 2 * WX; ly_ = 2 + (q2 >> 1); lx_ = q2 & 1; }
           const int kk = kfirst + pl;
-          load(std::false_type{}, gi0 + lx_, gj0 + ly_, kk, lx_ + WX * ly_, kk % 3, false);
+          load(std::false_type{}, gi0 + lx_, gj0 + ly_, kk, lx_ + WX * ly_, kk % RING, false);
         }
       }
-    }
-    for (int t = tid; t < NEW; t += NT3) {
-      const int ex = ex0 - 1 + t % (TX + 1), ey = ey0 - 1 + t / (TX + 1);
-      int ty_ = 0;
-      if (ex >= 0 && ey >= 0 && ex < g.nex && ey < g.ney) ty_ = rd3(g, ex, ey, ez) ? 1 : 2;
-      sType[t] = ty_;
     }
     __syncthreads();
 
@@ -422,14 +422,17 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
       if (useK) {
         for (int t = tid; t < NEW * 27; t += NT3) {
           const int e = t / 27, q = t - 27 * e;
-          if (sType[e] != 1) continue;
+          {
+            const int ex = ex0 - 1 + e % (TX + 1), ey = ey0 - 1 + e / (TX + 1);
+            if (ex < 0 || ey < 0 || ex >= g.nex || ey >= g.ney || !rd3(g, ex, ey, ez)) continue;
+          }
           const int lx = q % 3, ly = (q / 3) % 3, lz = q / 9;
           const int li0 = 2 * (e % (TX + 1)), lj0 = 2 * (e / (TX + 1));
           const int rq = (li0 + lx) + WX * (lj0 + ly);
           const double fx0 = sDM[3 * lx], fx1 = sDM[3 * lx + 1], fx2 = sDM[3 * lx + 2];
           const double fy0 = sDM[3 * ly], fy1 = sDM[3 * ly + 1], fy2 = sDM[3 * ly + 2];
           const double fz0 = sDM[3 * lz], fz1 = sDM[3 * lz + 1], fz2 = sDM[3 * lz + 2];
-          const int sq = (2 * ez + lz) % 3, z0 = (2 * ez) % 3, z1 = (2 * ez + 1) % 3, z2 = (2 * ez + 2) % 3;
+          const int sq = (2 * ez + lz) % RING, z0 = (2 * ez) % RING, z1 = (2 * ez + 1) % RING, z2 = (2 * ez + 2) % RING;
           double d[3][3];
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -449,8 +452,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
           sSig[e][5][q] = W * (mu * (d[1][2] + d[2][1]));
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
 
     // ------------------------------------------------------- owner phase
     if (p != 0 || first) {
@@ -460,8 +463,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
     const bool top_plane = (ez == g.nez - 1) && p == 2;
     if (owned && (!halo || p == 2)) {
       const int lz = p;
-      const int sP = k % 3;
-      const int z0 = (2 * ez) % 3, z1 = (2 * ez + 1) % 3, z2 = (2 * ez + 2) % 3;
+      const int sP = k % RING;
+      const int z0 = (2 * ez) % RING, z1 = (2 * ez + 1) % RING, z2 = (2 * ez + 2) % RING;
       const bool pn = HAS_PML && pml3(g, i, j, k);
       // z coefficients: forward DM(lz, .), transposed DM(., lz)
       const double fz0 = sDM[3 * lz], fz1 = sDM[3 * lz + 1], fz2 = sDM[3 * lz + 2];
@@ -479,7 +482,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
           else if (bb == 0) { if (j < 2) continue; ey = (j >> 1) - 1; ly = 2; }
           else { if ((j >> 1) >= g.ney) continue; ey = j >> 1; ly = 0; }
           const int e = (ex - ex0 + 1) + (TX + 1) * (ey - ey0 + 1);
-          const int et = sType[e];
+          const int et = rd3(g, ex, ey, ez) ? 1 : 2;
           const int rP = rown;                        // window position of (i, j)
           const double tx0 = sDM[lx], tx1 = sDM[3 + lx], tx2 = sDM[6 + lx];
           const double ty0 = sDM[ly], ty1 = sDM[3 + ly], ty2 = sDM[6 + ly];
@@ -615,7 +618,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
         }
       }
     }
-    __syncthreads();
+    if constexpr (RING == 3) __syncthreads();
   }
 }
 
