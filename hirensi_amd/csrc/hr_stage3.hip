@@ -49,6 +49,34 @@ __device__ __forceinline__ double ldo(const double* p, unsigned off) {
 __device__ __forceinline__ void sto(double* p, unsigned off, double v) {
   *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + off) = v;
 }
+// three consecutive components of a node (8-byte aligned): one 16-byte and one
+// 8-byte access instead of three 8-byte ones
+struct alignas(8) D3 {
+  double v[3];
+};
+__device__ __forceinline__ void ld3(const double* p, unsigned off, double& a, double& b, double& c) {
+  const D3 t = *reinterpret_cast<const D3*>(reinterpret_cast<const char*>(p) + off);
+  a = t.v[0];
+  b = t.v[1];
+  c = t.v[2];
+}
+__device__ __forceinline__ void st3(double* p, unsigned off, double a, double b, double c) {
+  D3 t;
+  t.v[0] = a;
+  t.v[1] = b;
+  t.v[2] = c;
+  *reinterpret_cast<D3*>(reinterpret_cast<char*>(p) + off) = t;
+}
+// stores of three consecutive components, each only where its bit of act is set
+__device__ __forceinline__ void st3m(double* p, unsigned off, int act, double a, double b, double c) {
+  if (act == 7) {
+    st3(p, off, a, b, c);
+  } else {
+    if (act & 1) sto(p, off, a);
+    if (act & 2) sto(p, off + 8u, b);
+    if (act & 4) sto(p, off + 16u, c);
+  }
+}
 
 __device__ __forceinline__ bool in_dom3(const Geo3& g, int i, int j, int k) {
   return i >= 0 && j >= 0 && k >= 0 && i < g.NX && j < g.NY && k < g.NZ;
@@ -228,12 +256,10 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
       // ---- u components: raw inputs
       double XU[3][NIN];
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int q = 0; q < NIN; ++q) {
-          const bool need = ALL_U || v2_input<S>(q);
-          XU[c][q] = need ? ldo(inp[q], offu + 8u * c) : 0.0;
-        }
+      for (int q = 0; q < NIN; ++q) {
+        if (ALL_U || v2_input<S>(q)) ld3(inp[q], offu, XU[0][q], XU[1][q], XU[2][q]);
+        else XU[0][q] = XU[1][q] = XU[2][q] = 0.0;
+      }
       if constexpr (HAS_RD) {
         lmv[0] = ldo(nodet, off + NT_LAM * NN8);
         lmv[1] = ldo(nodet, off + NT_MU * NN8);
@@ -244,13 +270,11 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
       for (int c = 0; c < 3; ++c) combine<S, NEG>(XU[c], A.hh, A.h, mask, v1u[c], vu2[c], v3u[c]);
       if constexpr (OWN && S == 4) {
         if (own_plane && ua) {
+          double xn0[3], xn1[3];
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            double xn0, xn1;
-            rk4_update12(XU[c], A.hh, A.h, A.h6, xn0, xn1, c3o[c]);
-            sto(xnp[0], offu + 8u * c, xn0);
-            sto(xnp[1], offu + 8u * c, xn1);
-          }
+          for (int c = 0; c < 3; ++c) rk4_update12(XU[c], A.hh, A.h, A.h6, xn0[c], xn1[c], c3o[c]);
+          st3(xnp[0], offu, xn0[0], xn0[1], xn0[2]);
+          st3(xnp[1], offu, xn1[0], xn1[1], xn1[2]);
         }
       }
       if constexpr (HAS_PML) {
@@ -312,9 +336,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
           for (int hf = 0; hf < 2; ++hf) {
             double XS[3][NIN];
 #pragma unroll
-            for (int t = 0; t < 3; ++t)
-#pragma unroll
-              for (int q = 0; q < NIN; ++q) XS[t][q] = ldo(inp[q], offs + 8u * (3 * hf + t));
+            for (int q = 0; q < NIN; ++q) ld3(inp[q], offs + 24u * hf, XS[0][q], XS[1][q], XS[2][q]);
             double s1[3], s2[3], s3[3];
 #pragma unroll
             for (int t = 0; t < 3; ++t) combine<S, NEG>(XS[t], A.hh, A.h, mask, s1[t], s2[t], s3[t]);
@@ -349,16 +371,17 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
               for (int t = 0; t < 3; ++t) dg[3 + 3 * hf + t] = diag(wm, s1[t], s2[t], s3[t]);
               if constexpr (S == 4) {
                 if (own_plane) {
+                  double xn0[3], xn1[3];
 #pragma unroll
                   for (int t = 0; t < 3; ++t) {
-                    const int m = 3 * hf + t;
-                    double xn0, xn1, c3v;
-                    rk4_update12(XS[t], A.hh, A.h, A.h6, xn0, xn1, c3v);
-                    if constexpr (NS == 9) c3o[3 + m] = c3v;
-                    if ((sm >> m) & 1) {
-                      sto(xnp[0], offs + 8u * m, xn0);
-                      sto(xnp[1], offs + 8u * m, xn1);
-                    }
+                    double c3v;
+                    rk4_update12(XS[t], A.hh, A.h, A.h6, xn0[t], xn1[t], c3v);
+                    if constexpr (NS == 9) c3o[3 + 3 * hf + t] = c3v;
+                  }
+                  const int act = (sm >> (3 * hf)) & 7;
+                  if (act) {
+                    st3m(xnp[0], offs + 24u * hf, act, xn0[0], xn0[1], xn0[2]);
+                    st3m(xnp[1], offs + 24u * hf, act, xn1[0], xn1[1], xn1[2]);
                   }
                 }
               }
@@ -561,27 +584,21 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
       if (!(p == 2 && !top_plane) && !halo) {
         const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
         const unsigned offu = n * 24u, offs = SB + n * 48u;
-        auto slot = [&](int m) { return m < 3 ? offu + 8u * m : offs + 8u * (m - 3); };
         const bool ua = u_act3(g, i, j, k);
         const int sm = pn ? smask3(g, i, j, k) : 0;
         double raw[9];
 #pragma unroll
         for (int m = 0; m < 9; ++m) raw[m] = (HAS_PML && pn) ? acc[m] + dg[m] : acc[m];
+        double out[9];
         if constexpr (S == 0) {
 #pragma unroll
-          for (int m = 0; m < 9; ++m) {
-            const bool act = m < 3 ? ua : ((sm >> (m - 3)) & 1);
-            if (act) sto(outp, slot(m), raw[m]);
-          }
+          for (int m = 0; m < 9; ++m) out[m] = raw[m];
         } else {
           const double minv_u = minvu_o, minv_s = minvs_o;
 #pragma unroll
           for (int m = 0; m < 9; ++m) {
-            const bool act = m < 3 ? ua : ((sm >> (m - 3)) & 1);
-            if (!act) continue;
-            if (!HAS_PML && m >= 3) continue;
             double src = 0.0;
-            if (m < 3 && A.src_mode) {
+            if (m < 3 && A.src_mode && ua) {
               const Src3* sp = A.src;
               const int tp = shot * sp->ntiles + b;
               const int j0 = sp->tile_ptr[tp], j1 = sp->tile_ptr[tp + 1];
@@ -609,12 +626,18 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
             const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
             const double gout = src - minv * raw[m];
             if constexpr (S < 4) {
-              sto(outp, slot(m), gout);
+              out[m] = gout;
             } else {
               const int cm = (NS == 9) ? m : (m < 3 ? m : 0);
-              sto(xnp[2], slot(m), c3o[cm] + A.h6 * gout);
+              out[m] = c3o[cm] + A.h6 * gout;
             }
           }
+        }
+        double* const dst = (S == 4) ? xnp[2] : outp;
+        if (ua) st3(dst, offu, out[0], out[1], out[2]);
+        if constexpr (HAS_PML) {
+          if (sm & 7) st3m(dst, offs, sm & 7, out[3], out[4], out[5]);
+          if (sm & 56) st3m(dst, offs + 24u, (sm >> 3) & 7, out[6], out[7], out[8]);
         }
       }
     }
