@@ -743,26 +743,28 @@ int hr_set_material(hr_ctx* ctx, const double* lam, const double* mu, const doub
         ctx->h_minv_u[n] = M > 0.0 ? 1.0 / M : 0.0;
       }
   ctx->h_wps = wps;
-  // v3 tables: node table [lam mu minv_u minv_s rwp wps] and [wq | per-axis stretch a, b]
+  // v3 tables: node table [(lam, mu) pairs | (minv_u, minv_s, rwp, wps) quadruples]
+  // and [wq | per-axis stretch (a, b) pairs]
   {
     std::vector<double> nt((size_t)NT_COUNT * NN, 0.0);
+    double* quad = nt.data() + (size_t)NT_PAIR * NN;
     for (size_t n = 0; n < NN; ++n) {
-      nt[NT_LAM * NN + n] = hl[n];
-      nt[NT_MU * NN + n] = hm[n];
-      nt[NT_MINV_U * NN + n] = ctx->h_minv_u[n];
-      nt[NT_MINV_S * NN + n] = wps[n] > 0.0 ? 1.0 / (ctx->h_a[n] * wps[n]) : 0.0;
-      nt[NT_RWP * NN + n] = rwp[n];
-      nt[NT_WPS * NN + n] = wps[n];
+      nt[2 * n] = hl[n];
+      nt[2 * n + 1] = hm[n];
+      quad[4 * n] = ctx->h_minv_u[n];
+      quad[4 * n + 1] = wps[n] > 0.0 ? 1.0 / (ctx->h_a[n] * wps[n]) : 0.0;
+      quad[4 * n + 2] = rwp[n];
+      quad[4 * n + 3] = wps[n];
     }
     if (int rc = upload(ctx, &ctx->d_node, nt)) return rc;
-    std::vector<double> tb(TAB_STRETCH + 2 * (size_t)(g.NX + g.NY + g.NZ));
+    std::vector<double> tb(TAB_STRETCH + 2 * (size_t)(g.NX + g.NY + g.NZ), 0.0);
     for (int q = 0; q < 27; ++q) tb[q] = g.wq[q];
     double* ax = tb.data() + TAB_STRETCH;
     double* ay = ax + 2 * g.NX;
     double* az = ay + 2 * g.NY;
-    for (int i = 0; i < g.NX; ++i) { double s[6]; h_stretch(g.pml, xc[i], yc[0], zc[g.NZ - 1], s); ax[i] = s[0]; ax[g.NX + i] = s[3]; }
-    for (int j = 0; j < g.NY; ++j) { double s[6]; h_stretch(g.pml, xc[0], yc[j], zc[g.NZ - 1], s); ay[j] = s[1]; ay[g.NY + j] = s[4]; }
-    for (int k = 0; k < g.NZ; ++k) { double s[6]; h_stretch(g.pml, xc[0], yc[0], zc[k], s); az[k] = s[2]; az[g.NZ + k] = s[5]; }
+    for (int i = 0; i < g.NX; ++i) { double s[6]; h_stretch(g.pml, xc[i], yc[0], zc[g.NZ - 1], s); ax[2 * i] = s[0]; ax[2 * i + 1] = s[3]; }
+    for (int j = 0; j < g.NY; ++j) { double s[6]; h_stretch(g.pml, xc[0], yc[j], zc[g.NZ - 1], s); ay[2 * j] = s[1]; ay[2 * j + 1] = s[4]; }
+    for (int k = 0; k < g.NZ; ++k) { double s[6]; h_stretch(g.pml, xc[0], yc[0], zc[k], s); az[2 * k] = s[2]; az[2 * k + 1] = s[5]; }
     if (int rc = upload(ctx, &ctx->d_tab, tb)) return rc;
   }
   if (int rc = upload(ctx, &ctx->d_lam, hl)) return rc;

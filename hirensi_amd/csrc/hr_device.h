@@ -109,10 +109,12 @@ struct Geo3 {
   int zb[HR_MAX_ZCHUNKS + 1];      // z-chunk c spans element layers [zb[c], zb[c+1])
 };
 
-// Node table: [lam | mu | minv_u | minv_s | rwp | wps], NN entries each.
-constexpr int NT_LAM = 0, NT_MU = 1, NT_MINV_U = 2, NT_MINV_S = 3, NT_RWP = 4, NT_WPS = 5, NT_COUNT = 6;
-// Small table: [wq (27) | ax bx (NX each) | ay by (NY each) | az bz (NZ each)].
-constexpr int TAB_STRETCH = 27;
+// Node table: pairs (lam, mu) per node (2 NN doubles), then quadruples
+// (minv_u, minv_s, rwp, wps) per node (4 NN doubles); each node's entries are
+// one or two 16-byte loads.
+constexpr int NT_PAIR = 2, NT_QUAD = 4, NT_COUNT = 6;
+// Small table: [wq (27) | pad | (a_x, b_x) pairs (NX) | (a_y, b_y) (NY) | (a_z, b_z) (NZ)].
+constexpr int TAB_STRETCH = 28;
 
 struct Src3 {                      // device-resident source entries of one solve, grouped by shot and tile
   const int* tile_ptr;             // (nshot * ntiles + 1): entries of (shot, tile) at [ptr[s*ntiles+t], ptr[..+1])

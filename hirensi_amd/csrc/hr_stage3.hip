@@ -49,6 +49,12 @@ __device__ __forceinline__ double ldo(const double* p, unsigned off) {
 __device__ __forceinline__ void sto(double* p, unsigned off, double v) {
   *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + off) = v;
 }
+// two consecutive doubles at a 16-byte aligned offset
+__device__ __forceinline__ void ld2(const double* p, unsigned off, double& a, double& b) {
+  const double2 t = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(p) + off);
+  a = t.x;
+  b = t.y;
+}
 // three consecutive components of a node (8-byte aligned): one 16-byte and one
 // 8-byte access instead of three 8-byte ones
 struct alignas(8) D3 {
@@ -202,7 +208,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   const int ez_first = max(ez0 - 1, 0);
   const int mask = (S == 0) ? A.set_mask : 7;
   const bool useG = mask & 1, useK = mask & 2, useC = mask & 4;
-  const unsigned NN8 = (unsigned)g.NN * 8u;   // node-table column stride in bytes
+  const unsigned NN8 = (unsigned)g.NN * 8u;   // bytes of one double per lattice node
   const unsigned SB = 3u * NN8;                 // byte offset of the stress block of a state vector
   const double* __restrict__ tab = A.tab;
   const double* __restrict__ nodet = A.node;
@@ -248,7 +254,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
     for (int m = 0; m < 9; ++m) { wv[m] = 0.0; Gv[m] = 0.0; }
     if (in_dom3(g, ii, jj, kk)) {
       const unsigned n = (unsigned)ii + (unsigned)g.NX * ((unsigned)jj + (unsigned)g.NY * (unsigned)kk);
-      const unsigned off = n * 8u;                 // node table
+      const unsigned offp = n * 16u;               // node table: (lam, mu)
+      const unsigned offq = 2u * NN8 + n * 32u;    // node table: (minv_u, minv_s, rwp, wps)
       const unsigned offu = n * 24u;               // displacement block of the state vectors
       const unsigned offs = SB + n * 48u;          // stress block
       const bool ua = u_act3(g, ii, jj, kk);
@@ -261,10 +268,10 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
         else XU[0][q] = XU[1][q] = XU[2][q] = 0.0;
       }
       if constexpr (HAS_RD) {
-        lmv[0] = ldo(nodet, off + NT_LAM * NN8);
-        lmv[1] = ldo(nodet, off + NT_MU * NN8);
+        ld2(nodet, offp, lmv[0], lmv[1]);
       }
-      if constexpr (OWN && S > 0) minvu_o = ldo(nodet, off + NT_MINV_U * NN8);
+      double minvs_l = 0.0;
+      if constexpr (OWN && S > 0) ld2(nodet, offq, minvu_o, minvs_l);
       double v1u[3], v3u[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) combine<S, NEG>(XU[c], A.hh, A.h, mask, v1u[c], vu2[c], v3u[c]);
@@ -283,19 +290,20 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
         // terms vanish), and w is never read (PML elements touch PML nodes only).
         {
           // ---- stretch factors, material
-          const unsigned ox = 8u * (unsigned)(TAB_STRETCH + ii), oy = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + jj);
-          const unsigned oz = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * g.NY + kk);
-          const double ax = ldo(tab, ox), bx = ldo(tab, ox + 8u * g.NX);
-          const double ay = ldo(tab, oy), by = ldo(tab, oy + 8u * g.NY);
-          const double az = ldo(tab, oz), bz = ldo(tab, oz + 8u * g.NZ);
+          const unsigned ox = 8u * (unsigned)(TAB_STRETCH + 2 * ii);
+          const unsigned oy = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * jj);
+          const unsigned oz = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * g.NY + 2 * kk);
+          double ax, bx, ay, by, az, bz;
+          ld2(tab, ox, ax, bx);
+          ld2(tab, oy, ay, by);
+          ld2(tab, oz, az, bz);
           double lam, mu;
           if constexpr (HAS_RD) { lam = lmv[0]; mu = lmv[1]; }
-          else { lam = ldo(nodet, off + NT_LAM * NN8); mu = ldo(nodet, off + NT_MU * NN8); }
+          else ld2(nodet, offp, lam, mu);
           double rwp = 0.0, wps = 0.0;
           if constexpr (OWN) {
-            rwp = ldo(nodet, off + NT_RWP * NN8);
-            wps = ldo(nodet, off + NT_WPS * NN8);
-            if constexpr (S > 0) minvs_o = pn ? ldo(nodet, off + NT_MINV_S * NN8) : 0.0;
+            ld2(nodet, offq + 16u, rwp, wps);
+            if constexpr (S > 0) minvs_o = pn ? minvs_l : 0.0;
           }
           // F[set][k], set 0 = w (G, v1), 1 = p (K, v2), 2 = e (C, v3)
           double F[3][3];
