@@ -285,10 +285,11 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
         }
       }
       if constexpr (HAS_PML) {
-        // Branch-free over the node class: at a node outside the PML the stress
-        // components are zero (so G = 0), b_x = b_y = b_z = 0 (so the diagonal
-        // terms vanish), and w is never read (PML elements touch PML nodes only).
-        {
+        // Only PML nodes (all window nodes of a PML-class tile): at any other node
+        // the stress components are zero (so G = 0), the diagonal terms vanish
+        // (b = 0) and w is never read, as PML elements touch PML nodes only.  The
+        // owner registers set here are read for PML nodes only.
+        if (HAS_RD ? pn : true) {
           // ---- stretch factors, material
           const unsigned ox = 8u * (unsigned)(TAB_STRETCH + 2 * ii);
           const unsigned oy = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * jj);
