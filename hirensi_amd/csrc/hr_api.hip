@@ -18,7 +18,7 @@
 using namespace hr;
 
 namespace {
-constexpr int DEFAULT_TZ = 8;
+constexpr int DEFAULT_TZ = 4;
 constexpr int DEFAULT_TY = 3;
 constexpr int DEFAULT_VERSION = 3;
 
