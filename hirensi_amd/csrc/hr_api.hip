@@ -321,7 +321,8 @@ Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
 
 int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (ctx->profiling) {
+  const bool prof = ctx->profiling && A.stage <= 4;   // per-kind statistics cover the apply and RK4 stages
+  if (prof) {
     if (ctx->ev_open.size() >= 4096) if (int rc = drain_events(ctx)) return rc;
     e0 = take_event(ctx);
     e1 = take_event(ctx);
@@ -354,7 +355,7 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
         HIPC(launch_stage3(B, A.stage, A.transpose, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
     }
   }
-  if (ctx->profiling) {
+  if (prof) {
     HIPC(hipEventRecord(e1, ctx->stream));
     ctx->ev_open.push_back({A.stage, {e0, e1}});
   }
@@ -881,9 +882,46 @@ int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   return setup_tiles(ctx);
 }
 
-int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double* ftab, int nsrc, const int* src_shot,
-                     const long long* src_eq, const double* src_val, int nrec, const long long* rec_eq, double* traces,
-                     int history, int segment) {
+}  // extern "C"
+
+namespace {
+
+// One step of the central-difference variant (EXPLICIT_Central_Difference_3,
+// Obsolete.F90:815-995) on the lattice state: x[1] = u_n, x[0] = u_{n-1},
+// x[2] = u_his; writes u_{n+1} to y[1] and u_his + dt u_{n+1} to y[2], then
+// rotates (u_{n-1}, u_n, u_his) <- (u_n, u_{n+1}, new u_his).
+int cd_step(hr_ctx* ctx, double dt, int n) {
+  StageArgs A = base_args(ctx);
+  A.stage = 5;
+  A.set_mask = 7;
+  A.hh = dt * dt; A.h = dt; A.h6 = dt * dt * dt;
+  A.in[0] = ctx->x[1]; A.in[1] = ctx->x[0]; A.in[2] = ctx->x[2];
+  A.xnew[0] = ctx->y[1]; A.xnew[1] = ctx->y[2];
+  Src3Spec sp;
+  if (ctx->d_src_tile_ptr) {
+    A.src.tile_ptr = ctx->d_src_tile_ptr;
+    A.nsrc_tiles = 1;
+    sp.dev = ctx->d_src3_fwd;
+    sp.mode = 1;
+    sp.step = n;
+    sp.fcol = 0;
+  }
+  if (int rc = launch_stage(ctx, A, ctx->d_src_tile_ptr ? &sp : nullptr)) return rc;
+  double* nm = ctx->x[0];
+  double* his = ctx->x[2];
+  ctx->x[0] = ctx->x[1];
+  ctx->x[1] = ctx->y[1];
+  ctx->x[2] = ctx->y[2];
+  ctx->y[1] = nm;
+  ctx->y[2] = his;
+  return 0;
+}
+
+// scheme 0: RK4 (hr_forward_shots); scheme 1: central difference (hr_forward_cd,
+// one shot, ftab already expanded to dt^2 f(n dt) in column 0 on the host)
+int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* ftab, int nsrc, const int* src_shot,
+                 const long long* src_eq, const double* src_val, int nrec, const long long* rec_eq, double* traces,
+                 int history, int segment, int scheme) {
   if (int rc = check_ready(ctx)) return rc;
   if (nshot < 1) return fail(ctx, 1, "nshot must be >= 1");
   if (nstep < 1 || !(dt > 0.0)) return fail(ctx, 1, "nstep must be >= 1 and dt > 0");
@@ -924,7 +962,7 @@ int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double*
   }
   const size_t nft = 3 * (size_t)nstep * nshot;
   ctx->ftab.resize(nft);
-  if (dev) HIPC(hipMemcpy(ctx->ftab.data(), ftab, nft * sizeof(double), hipMemcpyDeviceToHost));
+  if (dev && scheme == 0) HIPC(hipMemcpy(ctx->ftab.data(), ftab, nft * sizeof(double), hipMemcpyDeviceToHost));
   else std::memcpy(ctx->ftab.data(), ftab, nft * sizeof(double));
   if (ctx->ftab_cap < nft) {
     if (int rc = dalloc(ctx, &ctx->d_ftab, nft)) return rc;
@@ -992,6 +1030,20 @@ int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double*
     }
   }
   if (int rc = zero_state(ctx)) return rc;
+  if (scheme == 1) {
+    // traces row n = u at the start of step n (n >= 1; row 0 is u_0 = 0), as U_Store(:, ISTEP)
+    if (int rc = record_step(ctx, 0)) return rc;
+    for (int n = 1; n <= nstep; ++n) {
+      if (int rc = record_step(ctx, n)) return rc;
+      if (int rc = cd_step(ctx, dt, n - 1)) return rc;
+    }
+    if (traces && nrec)
+      HIPC(hipMemcpyAsync(traces, ctx->d_traces, rows * nrec * sizeof(double),
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    ctx->f_nstep = 0;                 // no history for an adjoint
+    return 0;
+  }
   const double sgn[3] = {1.0, 1.0, 1.0};
   const size_t nfs = field_size(ctx) * nshot;
   for (int n = 0; n < nstep; ++n) {
@@ -1017,6 +1069,31 @@ int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double*
   }
   HIPC(hipStreamSynchronize(ctx->stream));
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double* ftab, int nsrc, const int* src_shot,
+                     const long long* src_eq, const double* src_val, int nrec, const long long* rec_eq, double* traces,
+                     int history, int segment) {
+  return forward_impl(ctx, nshot, nstep, dt, ftab, nsrc, src_shot, src_eq, src_val, nrec, rec_eq, traces, history,
+                      segment, 0);
+}
+
+int hr_forward_cd(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, const long long* src_eq,
+                  const double* src_val, int nrec, const long long* rec_eq, double* traces) {
+  if (!ctx) return 1;
+  if (nstep < 1 || !(dt > 0.0)) return fail(ctx, 1, "nstep must be >= 1 and dt > 0");
+  if (!ftab) return fail(ctx, 1, "null time-function table");
+  // f(n dt), n = 1..nstep, into column 0 of step n - 1, scaled by dt^2 (B' = dt^2 M^-1 B)
+  std::vector<double> f(nstep), t3(3 * (size_t)nstep, 0.0);
+  if (ctx->flags & HR_DEVICE_PTRS) HIPC(hipMemcpy(f.data(), ftab, nstep * sizeof(double), hipMemcpyDeviceToHost));
+  else std::memcpy(f.data(), ftab, nstep * sizeof(double));
+  for (int n = 0; n < nstep; ++n) t3[3 * (size_t)n] = (dt * dt) * f[n];
+  return forward_impl(ctx, 1, nstep, dt, t3.data(), nsrc, nullptr, src_eq, src_val, nrec, rec_eq, traces,
+                      HR_HISTORY_NONE, 1, 1);
 }
 
 int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, const long long* src_eq,

@@ -106,10 +106,12 @@ __device__ __forceinline__ int sym3(int i, int k) { return i == k ? i : i + k + 
 
 // Stage-S inputs (v1, v2, v3) of one field component from its raw inputs X
 // (x1 x2 x3 g1 g2 g3; apply: x) in the reference's nested VecWAXPY order.
-// NEG: adjoint signs on v1, v3.
+// NEG: adjoint signs on v1, v3.  S = 5 is one step of the central-difference
+// variant (X = u_n, u_{n-1}, u_his; hh = dt^2, h = dt, h6 = dt^3): the G, K
+// and C inputs whose operator sum, scaled by -M^-1, is that step's update.
 template <int S, bool NEG>
-__device__ __forceinline__ void combine(const double* X, double hh, double h, int mask, double& v1, double& v2,
-                                        double& v3) {
+__device__ __forceinline__ void combine(const double* X, double hh, double h, double h6, int mask, double& v1,
+                                        double& v2, double& v3) {
   if constexpr (S == 0) {
     v1 = (mask & 1) ? X[0] : 0.0;
     v2 = (mask & 2) ? X[0] : 0.0;
@@ -121,6 +123,8 @@ __device__ __forceinline__ void combine(const double* X, double hh, double h, in
     v1 = X[0] + hh * X[1]; v2 = X[1] + hh * X[2]; v3 = X[2] + hh * X[3];
   } else if constexpr (S == 3) {
     v1 = X[0] + hh * (X[1] + hh * X[2]); v2 = X[1] + hh * (X[2] + hh * X[3]); v3 = X[2] + hh * X[4];
+  } else if constexpr (S == 5) {
+    v1 = h6 * X[0] + hh * X[2]; v2 = hh * X[0]; v3 = h * (X[0] - X[1]);
   } else {
     v1 = X[0] + h * (X[1] + hh * (X[2] + hh * X[3])); v2 = X[1] + h * (X[2] + hh * X[4]); v3 = X[2] + h * X[5];
   }
@@ -130,7 +134,7 @@ __device__ __forceinline__ void combine(const double* X, double hh, double h, in
 // raw input q enters the K-set input v2 of stage S
 template <int S>
 __device__ __forceinline__ constexpr bool v2_input(int q) {
-  return S == 0 ? q == 0 : (S == 1 ? q == 1 : (S == 2 ? (q == 1 || q == 2) : (S == 3 ? (q >= 1 && q <= 3) : (q == 1 || q == 2 || q == 4))));
+  return S == 0 ? q == 0 : (S == 1 ? q == 1 : (S == 2 ? (q == 1 || q == 2) : (S == 3 ? (q >= 1 && q <= 3) : (S == 5 ? q == 0 : (q == 1 || q == 2 || q == 4)))));
 }
 
 // Stage-4 update of x1 and x2 (no g4 needed), and the x3 carry
@@ -162,7 +166,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   constexpr int WX = SH::WX, WP = SH::WP, NEW = SH::NEW, OX = SH::OX, OC = SH::OC, HP = SH::HP;
   static_assert(OC <= 64 && NT3 == 256, "one owner plane group per wavefront, plus one helper wavefront");
   constexpr bool NEG = (S > 0) && TR;
-  constexpr int NS = (S == 4) ? (HAS_PML ? 9 : 3) : 1;   // stage-4 register carry width
+  constexpr bool UPD = S == 4 || S == 5;                   // the stage writes the new state
+  constexpr int NS = UPD ? (HAS_PML ? 9 : 3) : 1;          // register carry width of the update
   // LDS ring of lattice planes.  With five slots the two planes that the next
   // slab loads never hold a plane of the current slab, so a wavefront that has
   // finished the current slab can start loading the next one without waiting
@@ -232,21 +237,24 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   const bool oddx = i & 1, oddy = j & 1;
 
   // owner registers: accumulators, diagonal terms, stage-4 carry
-  double acc[9], dg[9], c3o[NS];
+  // c3o: stage 4 x3 + dt/6 (g1 + 2 g2 + 2 g3); stage 5 2 u_n - u_{n-1}.  chs: stage 5 u_his
+  double acc[9], dg[9], c3o[NS], chs[S == 5 ? NS : 1];
   double lam_o = 0.0, mu_o = 0.0, minvu_o = 0.0, minvs_o = 0.0;
 #pragma unroll
   for (int m = 0; m < 9; ++m) { acc[m] = 0.0; dg[m] = 0.0; }
 #pragma unroll
   for (int m = 0; m < NS; ++m) c3o[m] = 0.0;
+#pragma unroll
+  for (int m = 0; m < (S == 5 ? NS : 1); ++m) chs[m] = 0.0;
 
   // Load one lattice node into the window ring; OWN: also the owner registers.
   // The loads of a node class are issued unconditionally, so that they leave
   // in one batch: inactive lattice entries are zero in every state vector and
   // never written, and contribute nothing.  Activity only gates the stores.
-  constexpr int NIN = (S == 0) ? 1 : 2 + S;            // x1 x2 x3 g1 g2 g3 prefix (apply: x)
+  constexpr int NIN = (S == 0) ? 1 : (S == 5 ? 3 : 2 + S);   // x1 x2 x3 g1 g2 g3 prefix (apply: x)
   auto load = [&](auto own_tag, int ii, int jj, int kk, int r, int slot, bool own_plane) {
     constexpr bool OWN = decltype(own_tag)::value;
-    constexpr bool ALL_U = HAS_PML || (OWN && S == 4);   // v1 and v3 of u, or the stage-4 update
+    constexpr bool ALL_U = HAS_PML || (OWN && UPD);      // v1 and v3 of u, or the state update
     double vu2[3] = {0.0, 0.0, 0.0};
     double lmv[2] = {0.0, 0.0};
     double wv[9], Gv[9];
@@ -274,7 +282,14 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
       if constexpr (OWN && S > 0) ld2(nodet, offq, minvu_o, minvs_l);
       double v1u[3], v3u[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) combine<S, NEG>(XU[c], A.hh, A.h, mask, v1u[c], vu2[c], v3u[c]);
+      for (int c = 0; c < 3; ++c) combine<S, NEG>(XU[c], A.hh, A.h, A.h6, mask, v1u[c], vu2[c], v3u[c]);
+      if constexpr (OWN && S == 5) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          c3o[c] = 2.0 * XU[c][0] + (-1.0) * XU[c][1];
+          chs[c] = XU[c][2];
+        }
+      }
       if constexpr (OWN && S == 4) {
         if (own_plane && ua) {
           double xn0[3], xn1[3];
@@ -348,7 +363,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
             for (int q = 0; q < NIN; ++q) ld3(inp[q], offs + 24u * hf, XS[0][q], XS[1][q], XS[2][q]);
             double s1[3], s2[3], s3[3];
 #pragma unroll
-            for (int t = 0; t < 3; ++t) combine<S, NEG>(XS[t], A.hh, A.h, mask, s1[t], s2[t], s3[t]);
+            for (int t = 0; t < 3; ++t) combine<S, NEG>(XS[t], A.hh, A.h, A.h6, mask, s1[t], s2[t], s3[t]);
 #pragma unroll
             for (int X = 0; X < 3; ++X) {
               if (!(mask & (1 << X))) continue;
@@ -378,6 +393,13 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
               const double wm = hf == 0 ? wps : 2.0 * wps;
 #pragma unroll
               for (int t = 0; t < 3; ++t) dg[3 + 3 * hf + t] = diag(wm, s1[t], s2[t], s3[t]);
+              if constexpr (S == 5 && NS == 9) {
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                  c3o[3 + 3 * hf + t] = 2.0 * XS[t][0] + (-1.0) * XS[t][1];
+                  chs[3 + 3 * hf + t] = XS[t][2];
+                }
+              }
               if constexpr (S == 4) {
                 if (own_plane) {
                   double xn0[3], xn1[3];
@@ -598,7 +620,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
         double raw[9];
 #pragma unroll
         for (int m = 0; m < 9; ++m) raw[m] = (HAS_PML && pn) ? acc[m] + dg[m] : acc[m];
-        double out[9];
+        double out[9], his[S == 5 ? 9 : 1];
         if constexpr (S == 0) {
 #pragma unroll
           for (int m = 0; m < 9; ++m) out[m] = raw[m];
@@ -634,19 +656,30 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
             }
             const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
             const double gout = src - minv * raw[m];
-            if constexpr (S < 4) {
+            if constexpr (!UPD) {
               out[m] = gout;
-            } else {
+            } else if constexpr (S == 4) {
               const int cm = (NS == 9) ? m : (m < 3 ? m : 0);
               out[m] = c3o[cm] + A.h6 * gout;
+            } else {
+              const int cm = (NS == 9) ? m : (m < 3 ? m : 0);
+              out[m] = c3o[cm] + gout;                        // u_{n+1}
+              his[m] = chs[cm] + A.h * out[m];                // u_his + dt u_{n+1}
             }
           }
         }
-        double* const dst = (S == 4) ? xnp[2] : outp;
+        double* const dst = (S == 4) ? xnp[2] : (S == 5 ? xnp[0] : outp);
         if (ua) st3(dst, offu, out[0], out[1], out[2]);
         if constexpr (HAS_PML) {
           if (sm & 7) st3m(dst, offs, sm & 7, out[3], out[4], out[5]);
           if (sm & 56) st3m(dst, offs + 24u, (sm >> 3) & 7, out[6], out[7], out[8]);
+        }
+        if constexpr (S == 5) {
+          if (ua) st3(xnp[1], offu, his[0], his[1], his[2]);
+          if constexpr (HAS_PML) {
+            if (sm & 7) st3m(xnp[1], offs, sm & 7, his[3], his[4], his[5]);
+            if (sm & 56) st3m(xnp[1], offs + 24u, (sm >> 3) & 7, his[6], his[7], his[8]);
+          }
         }
       }
     }
@@ -681,6 +714,7 @@ hipError_t launch_stage3(const Stage3Args& A, int stage, int transpose, int cls,
     case 7: return launch3<3, true>(A, cls, tiles, ntl, s);
     case 8: return launch3<4, false>(A, cls, tiles, ntl, s);
     case 9: return launch3<4, true>(A, cls, tiles, ntl, s);
+    case 10: return launch3<5, false>(A, cls, tiles, ntl, s);
     default: return hipErrorInvalidValue;
   }
 }

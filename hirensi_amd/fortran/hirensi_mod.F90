@@ -3,6 +3,7 @@
 ! next to USE PARAMETERS; the calls replace the assembled-matrix path:
 !   ASSEM_*_PETSC_3D + MatAssembly      -> hr_ctx_create, hr_set_pml, hr_set_material
 !   EXPLICIT_RK4_PETSC_PML_3D           -> hr_forward
+!   EXPLICIT_Central_Difference_3       -> hr_forward_cd
 !   *_Adjoint_OtD + *_Control_OtD       -> hr_adjoint_gradient
 !   MatMultAdd / MatMultTransposeAdd    -> hr_apply (or the MatShell adapter)
 !   Assem_R_Mg_3D + iMg inversion       -> hr_gradient_mass
@@ -149,6 +150,17 @@ module hirensi
       real(c_double), intent(in) :: dis_meas(*)
       real(c_double), intent(out) :: misfit
     end function hr_misfit
+
+    integer(c_int) function hr_forward_cd(ctx, nstep, dt, ftab, nsrc, src_eq, src_val, nrec, rec_eq, traces) &
+        bind(C, name="hr_forward_cd")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nstep, nsrc, nrec
+      real(c_double), value :: dt
+      real(c_double), intent(in) :: ftab(*), src_val(*)
+      integer(c_long_long), intent(in) :: src_eq(*), rec_eq(*)
+      real(c_double), intent(out) :: traces(*)
+    end function hr_forward_cd
 
     integer(c_int) function hr_gradient_mass(ctx, imgrad, restriction) bind(C, name="hr_gradient_mass")
       import :: c_int, c_ptr, c_double

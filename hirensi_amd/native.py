@@ -22,6 +22,7 @@ EXPORTS = (
     "hr_reset_scaling", "hr_apply", "hr_time_function", "hr_forward", "hr_adjoint_gradient",
     "hr_misfit", "hr_recompute_ratio", "hr_set_tiling", "hr_set_profiling", "hr_get_kernel_stats",
     "hr_forward_shots", "hr_gradient_mass", "hr_regularization", "hr_regularize_gradient",
+    "hr_forward_cd",
 )
 
 
@@ -91,6 +92,7 @@ def load():
     L.hr_set_tiling.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.hr_set_profiling.argtypes = [_vp, C.c_int]
     L.hr_get_kernel_stats.argtypes = [_vp, _dp, _dp]
+    L.hr_forward_cd.argtypes = [_vp, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp, C.c_int, _dp, _dp]
     L.hr_gradient_mass.argtypes = [_vp, _dp, _dp]
     L.hr_regularization.argtypes = [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     L.hr_regularize_gradient.argtypes = [_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp]

@@ -14,6 +14,8 @@ Names follow the reference (step_4_3D_Forward_and_Inverse/):
 * :func:`explicit_rk4_petsc_pml_3d_adjoint_otd` -- adjoint, control pass and
   sign flip (Solver_PETSc_Explicit.F90:1125-1522, Program_Main.F90:400-413).
 * :func:`compute_misfit` -- Compute_Misfit (Solver_PETSc_Inverse.F90:591-650).
+* :func:`explicit_central_difference_3` -- the central-difference variant
+  EXPLICIT_Central_Difference_3 (Obsolete.F90:815-995), forward only.
 * :func:`gradient_mass`, :func:`regularization`, :func:`regularize_gradient` --
   the gradient post-processing of the inversion loop: Assem_R_Mg_3D and its
   inversion (Assemble_PETSc_3D.F90:670-748, Control_Problem_Operators.F90:1-24),
@@ -35,7 +37,8 @@ from .native import (HrError, HR_K, HR_C, HR_G, HR_HISTORY_NONE, HR_HISTORY_STOR
 
 __all__ = ["Context", "ShellMat", "time_function", "time_table", "explicit_rk4_petsc_pml_3d",
            "explicit_rk4_petsc_pml_3d_shots",
-           "explicit_rk4_petsc_pml_3d_adjoint_otd", "compute_misfit", "gradient_mass", "regularization",
+           "explicit_rk4_petsc_pml_3d_adjoint_otd", "compute_misfit", "explicit_central_difference_3",
+           "gradient_mass", "regularization",
            "regularize_gradient", "HrError", "HR_REG_TV", "HR_REG_TN",
            "HR_K", "HR_C", "HR_G", "HR_HISTORY_NONE", "HR_HISTORY_STORE", "HR_HISTORY_CHECKPOINT"]
 
@@ -261,6 +264,26 @@ def explicit_rk4_petsc_pml_3d_shots(ctx: Context, nstep, dt, ftab, src_shot, src
                                       ptrs[1] if nsrc else None, ptrs[2] if nsrc else None,
                                       ptrs[3] if nsrc else None, nrec, ptrs[4] if nrec else None,
                                       _ptr(traces) if nrec else None, int(history), int(segment)))
+    return traces
+
+
+def explicit_central_difference_3(ctx: Context, nstep, dt, src_eq, src_val, rec_eq, lcase=2, fr=15.0, ftab=None):
+    """Central-difference forward solve (``hr_forward_cd``).  ``ftab`` (length
+    nstep) holds f(n dt) for n = 1..nstep; by default TIME_FUNCTION(lcase, fr).
+    Returns traces (nstep + 1, nrec): row n = u at the start of step n."""
+    if ctx.device_ptrs:
+        raise HrError("explicit_central_difference_3 takes host arrays; use a host-pointer context")
+    if ftab is None:
+        ftab = np.array([time_function(n * dt, lcase, fr) for n in range(1, nstep + 1)])
+    ftab = _f64(ftab)
+    if ftab.shape != (nstep,):
+        raise ValueError("ftab must hold nstep values")
+    src_eq, src_val, rec_eq = _i64(src_eq), _f64(src_val), _i64(rec_eq)
+    nsrc, nrec = len(src_eq), len(rec_eq)
+    traces = np.zeros((nstep + 1, nrec))
+    ctx._check(ctx.L.hr_forward_cd(ctx.h, int(nstep), float(dt), ftab.ctypes.data, nsrc,
+                                   src_eq.ctypes.data if nsrc else None, src_val.ctypes.data if nsrc else None,
+                                   nrec, rec_eq.ctypes.data if nrec else None, traces.ctypes.data if nrec else None))
     return traces
 
 

@@ -117,6 +117,19 @@ int hr_forward_shots(hr_ctx* ctx, int nshot, int nstep, double dt, const double*
                      const int* src_shot, const long long* src_eq, const double* src_val, int nrec,
                      const long long* rec_eq, double* traces, int history, int segment);
 
+/* The reference's central-difference variant EXPLICIT_Central_Difference_3
+ * (Obsolete.F90:815-995) on the same operators, one shot:
+ *   u_{n+1} = 2 u_n - u_{n-1} - M^-1 [dt^2 K u_n + dt C (u_n - u_{n-1})
+ *             + dt^3 G u_n + dt^2 G u_his] + dt^2 f((n+1) dt) M^-1 B,
+ *   u_his <- u_his + dt u_{n+1}, from zero initial state.
+ *   ftab[nstep]: f(n dt) for n = 1..nstep (TIME_FUNCTION at T = ISTEP dt)
+ *   traces[(nstep+1) x nrec]: row n = u at the start of step ISTEP = n
+ *   (U_Store(:, ISTEP)); row 0 is the zero initial state (the reference leaves
+ *   column 0 unset).  Leaves no history: hr_adjoint_gradient and hr_misfit
+ *   need an RK4 forward. */
+int hr_forward_cd(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, const long long* src_eq,
+                  const double* src_val, int nrec, const long long* rec_eq, double* traces);
+
 /* OtD adjoint + gradient (Adjoint_OtD, Control_OtD, g <- -g).  Uses the
  * history of the last hr_forward / hr_forward_shots.  dis_meas[(nstep+1) x
  * nrec] measured data per shot, shots one after another.

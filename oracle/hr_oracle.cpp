@@ -39,6 +39,7 @@
 #include <cstring>
 #include <cstdio>
 #include <vector>
+#include <map>
 #include <algorithm>
 #include <numeric>
 #ifdef _OPENMP
@@ -1084,6 +1085,67 @@ int or_rk4_forward(void* s, const double* Bin, int nstep, double dt, int lcase, 
     axpy(x2, dt / 6.0, help);
     maxpy4(help, k1b, k2b, k3b, k4b);
     axpy(x3, dt / 6.0, help);
+  }
+  return 0;
+}
+
+// EXPLICIT_Central_Difference_3 (Obsolete.F90:815-995): the reference's
+// central-difference variant on the same K, C, G.  Set-up: M^-1 row scaling,
+// K' = -dt^2 K - dt C - dt^3 G (MatAXPY, merged patterns), C' = dt C,
+// G' = -dt^2 G, B' = dt^2 M^-1 B.  Step ISTEP = 1..NSTEP at T = ISTEP dt:
+// U_Store(:, ISTEP) = u_nn, u_np = f(T) B' + (2 u_nn - u_nm) + K' u_nn + C' u_nm
+// + G' u_his, u_his += dt u_np, (u_nm, u_nn) <- (u_nn, u_np).  Row 0 of
+// u_store is left at zero (the reference never writes column 0).
+int or_cd_forward(void* s, const double* Bin, int nstep, double dt, int lcase, double fr, i64 nstore,
+                  const i64* store_eq, double* u_store) {
+  System* S = (System*)s;
+  const i64 n = S->neq;
+  const CSR &K = S->A[0], &C = S->A[1], &G = S->A[2];
+  std::vector<double> iM(n);
+  for (i64 i = 0; i < n; ++i) iM[i] = 1.0 / S->diag_m[i];
+  const double dt2 = dt * dt, dt3 = dt * dt * dt;
+  // merged pattern of K, C, G with K' values (MatScale then two MatAXPY)
+  CSR Kp;
+  Kp.n = n;
+  Kp.rowptr.assign(n + 1, 0);
+  std::vector<double> kv;
+  for (i64 i = 0; i < n; ++i) {
+    std::map<int, double> row;
+    for (i64 k = K.rowptr[i]; k < K.rowptr[i + 1]; ++k) row[K.col[k]] = (K.val[k] * iM[i]) * (-dt2);
+    for (i64 k = C.rowptr[i]; k < C.rowptr[i + 1]; ++k) {
+      auto it = row.find(C.col[k]);
+      const double a = (-dt) * (C.val[k] * iM[i]);
+      if (it == row.end()) row[C.col[k]] = 0.0 + a; else it->second = it->second + a;
+    }
+    for (i64 k = G.rowptr[i]; k < G.rowptr[i + 1]; ++k) {
+      auto it = row.find(G.col[k]);
+      const double a = (-dt3) * (G.val[k] * iM[i]);
+      if (it == row.end()) row[G.col[k]] = 0.0 + a; else it->second = it->second + a;
+    }
+    for (auto& kvp : row) { Kp.col.push_back(kvp.first); kv.push_back(kvp.second); }
+    Kp.rowptr[i + 1] = (i64)Kp.col.size();
+  }
+  std::vector<double> cv(C.val.size()), gv(G.val.size());
+  for (i64 i = 0; i < n; ++i) {
+    for (i64 k = C.rowptr[i]; k < C.rowptr[i + 1]; ++k) cv[k] = (C.val[k] * iM[i]) * dt;
+    for (i64 k = G.rowptr[i]; k < G.rowptr[i + 1]; ++k) gv[k] = (G.val[k] * iM[i]) * (-dt2);
+  }
+  std::vector<double> B(n);
+  for (i64 i = 0; i < n; ++i) B[i] = (Bin[i] * iM[i]) * dt2;
+  std::vector<double> u_nm(n, 0.0), u_nn(n, 0.0), u_np(n, 0.0), u_his(n, 0.0);
+  for (i64 r = 0; r < nstore; ++r) u_store[r] = 0.0;
+  for (int istep = 1; istep <= nstep; ++istep) {
+    const double T = (double)istep * dt;
+    for (i64 r = 0; r < nstore; ++r) u_store[r + nstore * (i64)istep] = u_nn[store_eq[r]];
+    const double fac = time_function(T, lcase, fr);
+    for (i64 i = 0; i < n; ++i) u_np[i] = 0.0 + fac * B[i];
+    for (i64 i = 0; i < n; ++i) u_np[i] = u_np[i] + (2.0 * u_nn[i] + (-1.0) * u_nm[i]);
+    matmult_add(Kp, kv.data(), u_nn.data(), u_np.data(), u_np.data());
+    matmult_add(C, cv.data(), u_nm.data(), u_np.data(), u_np.data());
+    matmult_add(G, gv.data(), u_his.data(), u_np.data(), u_np.data());
+    axpy(u_his, dt, u_np);
+    u_nm.swap(u_nn);
+    u_nn.swap(u_np);
   }
   return 0;
 }

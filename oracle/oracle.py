@@ -53,6 +53,8 @@ def lib():
         L.or_matmult_add.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.or_rk4_forward.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_int, C.c_double,
                                      C.c_int64, _ip, _dp]
+        L.or_cd_forward.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_int, C.c_double,
+                                    C.c_int64, _ip, _dp]
         L.or_rk4_adjoint.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int64, _ip, _dp,
                                      C.c_int, _ip, _ip, _dp, _dp]
         L.or_control.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int64, _ip, _dp, _dp, _dp, _dp]
@@ -292,6 +294,17 @@ class System:
         lib().or_rk4_forward(self.h, np.ascontiguousarray(B, dtype=np.float64), int(nstep), float(dt),
                              int(lcase), float(fr), ns, store_eq, u)
         return u.reshape(nstep + 1, ns)              # row n = U_Store(:, n)
+
+    def cd_forward(self, B, nstep, dt, lcase=2, fr=15.0, store_eq=None):
+        """EXPLICIT_Central_Difference_3: rows n = 1..nstep hold u at the start
+        of step n (row 0 stays zero)."""
+        store_eq = self.mesh.store_eq if store_eq is None else store_eq
+        store_eq = np.ascontiguousarray(store_eq, dtype=np.int64)
+        ns = len(store_eq)
+        u = np.zeros((nstep + 1) * ns)
+        lib().or_cd_forward(self.h, np.ascontiguousarray(B, dtype=np.float64), int(nstep), float(dt),
+                            int(lcase), float(fr), ns, store_eq, u)
+        return u.reshape(nstep + 1, ns)
 
     def adjoint(self, u_store, nstep, dt, rec_eq, dis_meas, store_eq=None):
         store_eq = self.mesh.store_eq if store_eq is None else store_eq
