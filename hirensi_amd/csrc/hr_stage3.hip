@@ -238,10 +238,11 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 
   // owner registers: accumulators, diagonal terms, stage-4 carry
   // c3o: stage 4 x3 + dt/6 (g1 + 2 g2 + 2 g3); stage 5 2 u_n - u_{n-1}.  chs: stage 5 u_his
-  double acc[9], dg[9], c3o[NS], chs[S == 5 ? NS : 1];
+  // acc starts at the node's diagonal terms (set when the owner loads its node)
+  double acc[9], c3o[NS], chs[S == 5 ? NS : 1];
   double lam_o = 0.0, mu_o = 0.0, minvu_o = 0.0, minvs_o = 0.0;
 #pragma unroll
-  for (int m = 0; m < 9; ++m) { acc[m] = 0.0; dg[m] = 0.0; }
+  for (int m = 0; m < 9; ++m) acc[m] = 0.0;
 #pragma unroll
   for (int m = 0; m < NS; ++m) c3o[m] = 0.0;
 #pragma unroll
@@ -255,6 +256,11 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   auto load = [&](auto own_tag, int ii, int jj, int kk, int r, int slot, bool own_plane) {
     constexpr bool OWN = decltype(own_tag)::value;
     constexpr bool ALL_U = HAS_PML || (OWN && UPD);      // v1 and v3 of u, or the state update
+    if constexpr (OWN) {
+      // a newly loaded owned node starts its sums at its diagonal terms (zero off the PML)
+#pragma unroll
+      for (int m = 0; m < 9; ++m) acc[m] = 0.0;
+    }
     double vu2[3] = {0.0, 0.0, 0.0};
     double lmv[2] = {0.0, 0.0};
     double wv[9], Gv[9];
@@ -349,7 +355,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
           };
           if constexpr (OWN) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) dg[c] = diag(rwp, v1u[c], vu2[c], v3u[c]);
+            for (int c = 0; c < 3; ++c) acc[c] = diag(rwp, v1u[c], vu2[c], v3u[c]);
             lam_o = lam;
             mu_o = mu;
           }
@@ -392,7 +398,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
             if constexpr (OWN) {
               const double wm = hf == 0 ? wps : 2.0 * wps;
 #pragma unroll
-              for (int t = 0; t < 3; ++t) dg[3 + 3 * hf + t] = diag(wm, s1[t], s2[t], s3[t]);
+              for (int t = 0; t < 3; ++t) acc[3 + 3 * hf + t] = diag(wm, s1[t], s2[t], s3[t]);
               if constexpr (S == 5 && NS == 9) {
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
@@ -510,10 +516,6 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
     }
 
     // ------------------------------------------------------- owner phase
-    if (p != 0 || first) {
-#pragma unroll
-      for (int m = 0; m < 9; ++m) acc[m] = 0.0;
-    }
     const bool top_plane = (ez == g.nez - 1) && p == 2;
     if (owned && (!halo || p == 2)) {
       const int lz = p;
@@ -619,7 +621,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
         const int sm = pn ? smask3(g, i, j, k) : 0;
         double raw[9];
 #pragma unroll
-        for (int m = 0; m < 9; ++m) raw[m] = (HAS_PML && pn) ? acc[m] + dg[m] : acc[m];
+        for (int m = 0; m < 9; ++m) raw[m] = acc[m];
         double out[9], his[S == 5 ? 9 : 1];
         if constexpr (S == 0) {
 #pragma unroll
