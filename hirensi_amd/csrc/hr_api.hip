@@ -559,8 +559,9 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
   *out = ctx;
   if (!d) return fail(ctx, 1, "null mesh descriptor");
   if (d->nex < 2 || d->ney < 2 || d->nez < 2) return fail(ctx, 1, "mesh must have at least 2 elements per axis");
+  // PML_PARAM divides by the layer thickness (PML_FINDER_3D), so every side needs a layer
   for (int a = 0; a < 5; ++a)
-    if (d->npml[a] < 0) return fail(ctx, 1, "negative PML thickness");
+    if (d->npml[a] < 1) return fail(ctx, 1, "every side needs at least one PML element layer");
   if (d->npml[0] + d->npml[1] >= d->nex || d->npml[2] + d->npml[3] >= d->ney || d->npml[4] >= d->nez)
     return fail(ctx, 1, "PML layers leave no regular domain");
   if (!(d->h[0] > 0 && d->h[1] > 0 && d->h[2] > 0)) return fail(ctx, 1, "element edges must be positive");

@@ -71,6 +71,9 @@ class BoxMesh:
         if self.top_stress_bc not in ("reference", "none"):
             raise ValueError("top_stress_bc must be 'reference' or 'none'")
         px0, px1, py0, py1, pz0 = self.npml
+        if min(self.npml) < 1:
+            # PML_PARAM holds the layer thickness as a divisor (PML_FINDER_3D)
+            raise ValueError("every side needs at least one PML element layer")
         if px0 + px1 >= self.nex or py0 + py1 >= self.ney or pz0 >= self.nez:
             raise ValueError("PML layers leave no regular domain")
         self.NX, self.NY, self.NZ = 2 * self.nex + 1, 2 * self.ney + 1, 2 * self.nez + 1

@@ -52,7 +52,7 @@ typedef struct hr_ctx hr_ctx;
  * (step_1_Ansys_Mesh_Generator, step_2_20_To_27/Main.F90:447-521). */
 typedef struct hr_mesh_desc {
   int nex, ney, nez;        /* elements per axis, PML included */
-  int npml[5];              /* PML element layers: x-, x+, y-, y+, z- (top is free) */
+  int npml[5];              /* PML element layers (>= 1): x-, x+, y-, y+, z- (top is free) */
   double h[3];              /* element edge lengths */
   int top_stress_bc;        /* 1: s_zz, s_xz, s_yz fixed at original top PML nodes */
   int flags;                /* HR_DEVICE_PTRS */
