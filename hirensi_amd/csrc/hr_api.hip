@@ -53,7 +53,6 @@ struct hr_ctx {
   long long nbox = 0;
   // node data (device)
   double *d_xc = nullptr, *d_yc = nullptr, *d_zc = nullptr;
-  double *d_lam = nullptr, *d_mu = nullptr, *d_minv = nullptr, *d_rwp = nullptr, *d_wps = nullptr;
   double* d_node = nullptr;                 // v3 node table (NT_COUNT x NN)
   double* d_tab = nullptr;                  // v3 small table (weights, per-axis stretch)
   Src3* d_src3_fwd = nullptr;               // v3 source lists (device structs)
@@ -366,14 +365,11 @@ StageArgs base_args(hr_ctx* ctx) {
   StageArgs A;
   std::memset(&A, 0, sizeof(A));
   A.g = ctx->g;
-  A.nd.xc = ctx->d_xc; A.nd.yc = ctx->d_yc; A.nd.zc = ctx->d_zc;
-  A.nd.lam = ctx->d_lam; A.nd.mu = ctx->d_mu;
-  A.nd.minv_u = ctx->d_minv; A.nd.rwp = ctx->d_rwp; A.nd.wps = ctx->d_wps;
   A.sgn[0] = A.sgn[1] = A.sgn[2] = 1.0;
   return A;
 }
 
-// Tile grid and the three tile-class lists of the v2 stage kernel.  A tile's
+// Tile grid and the three tile-class lists of the stage kernel.  A tile's
 // window is its elements plus the low-side halo element in x, y and z.
 int setup_tiles(hr_ctx* ctx) {
   Geo& g = ctx->g;
@@ -449,8 +445,7 @@ int zero_state(hr_ctx* ctx) {
 }
 
 // One RK4 step of every shot on (x) -> (y), then swap.  dir = +1 forward, -1 adjoint.
-int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3], const Source* src4,
-             const Src3Spec* sp4) {
+int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3], const Src3Spec* sp4) {
   const double hh = dir * (0.5 * dt), h = dir * dt, h6 = dir * (dt / 6.0);
   for (int s = 1; s <= 4; ++s) {
     StageArgs A = base_args(ctx);
@@ -463,7 +458,7 @@ int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3]
     A.in[3] = ctx->gs[0]; A.in[4] = ctx->gs[1]; A.in[5] = ctx->gs[2];
     if (s < 4) A.out = ctx->gs[s - 1];
     else { A.xnew[0] = ctx->y[0]; A.xnew[1] = ctx->y[1]; A.xnew[2] = ctx->y[2]; }
-    if (src4 && src4[s - 1].tile_ptr) { A.src = src4[s - 1]; A.nsrc_tiles = 1; }
+    if (sp4 && sp4[s - 1].mode) A.nsrc_tiles = 1;
     if (int rc = launch_stage(ctx, A, sp4 ? &sp4[s - 1] : nullptr)) return rc;
   }
   std::swap(ctx->x, ctx->y);
@@ -490,23 +485,14 @@ int store_slice(hr_ctx* ctx, double* base, size_t rows, size_t row) {
 }
 
 // forward sources of step n: B values times f(T), f(T + dt/2), f(T + dt)
-void forward_sources(hr_ctx* ctx, int n, Source s4[4], Src3Spec sp4[4]) {
+void forward_sources(hr_ctx* ctx, int n, Src3Spec sp4[4]) {
   for (int s = 0; s < 4; ++s) {
-    std::memset(&s4[s], 0, sizeof(Source));
     sp4[s] = Src3Spec();
     if (!ctx->d_src_tile_ptr) continue;
-    const int col = (s == 0) ? 0 : (s == 3 ? 2 : 1);
-    // single-shot v1/v2 kernels take f from the host table
-    s4[s].tile_ptr = ctx->d_src_tile_ptr;
-    s4[s].node = ctx->d_src_node;
-    s4[s].comp = ctx->d_src_comp;
-    s4[s].m = ctx->d_src_m;
-    s4[s].a = ctx->d_src_val;
-    s4[s].ca = ctx->ftab[3 * (size_t)n + col];
     sp4[s].dev = ctx->d_src3_fwd;
     sp4[s].mode = 1;
     sp4[s].step = n;
-    sp4[s].fcol = col;
+    sp4[s].fcol = (s == 0) ? 0 : (s == 3 ? 2 : 1);
   }
 }
 
@@ -515,10 +501,9 @@ int run_forward_steps(hr_ctx* ctx, int n0, int n1, double* seg_hist, size_t seg_
   const double sgn[3] = {1.0, 1.0, 1.0};
   for (int n = n0; n < n1; ++n) {
     if (seg_hist) if (int rc = store_slice(ctx, seg_hist, seg_rows, (size_t)(n - n0))) return rc;
-    Source s4[4];
     Src3Spec sp4[4];
-    forward_sources(ctx, n, s4, sp4);
-    if (int rc = rk4_step(ctx, ctx->f_dt, +1, 0, sgn, s4, sp4)) return rc;
+    forward_sources(ctx, n, sp4);
+    if (int rc = rk4_step(ctx, ctx->f_dt, +1, 0, sgn, sp4)) return rc;
   }
   if (seg_hist) if (int rc = store_slice(ctx, seg_hist, seg_rows, (size_t)(n1 - n0))) return rc;
   return 0;
@@ -636,8 +621,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
 int hr_ctx_destroy(hr_ctx* ctx) {
   if (!ctx) return 0;
   auto F = [](void* p) { if (p) (void)hipFree(p); };
-  F(ctx->d_eq_slot); F(ctx->d_xc); F(ctx->d_yc); F(ctx->d_zc); F(ctx->d_lam); F(ctx->d_mu); F(ctx->d_minv);
-  F(ctx->d_rwp); F(ctx->d_wps);
+  F(ctx->d_eq_slot); F(ctx->d_xc); F(ctx->d_yc); F(ctx->d_zc);
   F(ctx->d_node); F(ctx->d_tab); F(ctx->d_src3_fwd); F(ctx->d_src3_adj); F(ctx->d_ftab);
   for (int a = 0; a < 3; ++a) { F(ctx->x[a]); F(ctx->y[a]); F(ctx->gs[a]); F(ctx->shell[a].l); F(ctx->shell[a].r); }
   F(ctx->tmp); F(ctx->tmp2); F(ctx->d_traces); F(ctx->d_rec_slot); F(ctx->d_hist); F(ctx->d_seg);
@@ -772,11 +756,6 @@ int hr_set_material(hr_ctx* ctx, const double* lam, const double* mu, const doub
     for (int k = 0; k < g.NZ; ++k) { double s[6]; h_stretch(g.pml, xc[0], yc[0], zc[k], s); az[2 * k] = s[2]; az[2 * k + 1] = s[5]; }
     if (int rc = upload(ctx, &ctx->d_tab, tb)) return rc;
   }
-  if (int rc = upload(ctx, &ctx->d_lam, hl)) return rc;
-  if (int rc = upload(ctx, &ctx->d_mu, hm)) return rc;
-  if (int rc = upload(ctx, &ctx->d_minv, ctx->h_minv_u)) return rc;
-  if (int rc = upload(ctx, &ctx->d_rwp, rwp)) return rc;
-  if (int rc = upload(ctx, &ctx->d_wps, wps)) return rc;
   ctx->have_material = true;
   return 0;
 }
@@ -903,7 +882,6 @@ int cd_step(hr_ctx* ctx, double dt, int n) {
   A.xnew[0] = ctx->y[1]; A.xnew[1] = ctx->y[2];
   Src3Spec sp;
   if (ctx->d_src_tile_ptr) {
-    A.src.tile_ptr = ctx->d_src_tile_ptr;
     A.nsrc_tiles = 1;
     sp.dev = ctx->d_src3_fwd;
     sp.mode = 1;
@@ -1060,10 +1038,9 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
       for (int a = 0; a < 3; ++a)
         HIPC(hipMemcpyAsync(cp + a * nfs, ctx->x[a], nfs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     }
-    Source s4[4];
     Src3Spec sp4[4];
-    forward_sources(ctx, n, s4, sp4);
-    if (int rc = rk4_step(ctx, dt, +1, 0, sgn, s4, sp4)) return rc;
+    forward_sources(ctx, n, sp4);
+    if (int rc = rk4_step(ctx, dt, +1, 0, sgn, sp4)) return rc;
   }
   if (int rc = record_step(ctx, nstep)) return rc;
   if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist, rows, (size_t)nstep)) return rc;
@@ -1279,10 +1256,8 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
                              dt, ctx->d_gl, ctx->d_gm, ctx->stream));
     }
     // misfit sources at t_n (stage 1), t_{n-1} (stage 4), mean (stages 2, 3)
-    Source s4[4];
     Src3Spec sp4[4];
     for (int s = 0; s < 4; ++s) {
-      std::memset(&s4[s], 0, sizeof(Source));
       sp4[s] = Src3Spec();
       if (!nrec) continue;
       sp4[s].dev = ctx->d_src3_adj;
@@ -1290,24 +1265,12 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
       if (s == 0) { sp4[s].ra = n; sp4[s].ca = 1.0; }
       else if (s == 3) { sp4[s].ra = n - 1; sp4[s].ca = 1.0; }
       else { sp4[s].ra = n; sp4[s].ca = 0.5; sp4[s].rb = n - 1; sp4[s].cb = 0.5; }
-      // single-shot v1/v2 kernels
-      s4[s].tile_ptr = ctx->d_rec_tile_ptr;
-      s4[s].node = ctx->d_rec_node;
-      s4[s].comp = ctx->d_rec_comp;
-      s4[s].m = ctx->d_rec_m;
-      const double* tn = ctx->d_tr_sorted + (size_t)n * nrec;
-      const double* on = ctx->d_obs_sorted + (size_t)n * nrec;
-      const double* tp = ctx->d_tr_sorted + (size_t)(n - 1) * nrec;
-      const double* op = ctx->d_obs_sorted + (size_t)(n - 1) * nrec;
-      if (s == 0) { s4[s].a = tn; s4[s].a2 = on; s4[s].ca = 1.0; }
-      else if (s == 3) { s4[s].a = tp; s4[s].a2 = op; s4[s].ca = 1.0; }
-      else { s4[s].a = tn; s4[s].a2 = on; s4[s].ca = 0.5; s4[s].b = tp; s4[s].b2 = op; s4[s].cb = 0.5; }
     }
     // adjoint step on adj -> adjy
     double* fx[3] = {ctx->x[0], ctx->x[1], ctx->x[2]};
     double* fy[3] = {ctx->y[0], ctx->y[1], ctx->y[2]};
     for (int a = 0; a < 3; ++a) { ctx->x[a] = adj[a]; ctx->y[a] = adjy[a]; }
-    if (int rc = rk4_step(ctx, dt, -1, 1, sgn, s4, sp4)) { status = rc; }
+    if (int rc = rk4_step(ctx, dt, -1, 1, sgn, sp4)) { status = rc; }
     for (int a = 0; a < 3; ++a) { adj[a] = ctx->x[a]; adjy[a] = ctx->y[a]; }
     if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
       for (int a = 0; a < 3; ++a) { ctx->x[a] = fx[a]; ctx->y[a] = fy[a]; }

@@ -22,38 +22,16 @@ struct Geo {
   double pml[24];                  // PML_PARAM(6,4), column-major
 };
 
-// Per-node arrays on the lattice (length NN).
-struct NodeData {
-  const double *xc, *yc, *zc;      // lattice coordinates per axis
-  const double *lam, *mu;          // nodal Lame parameters
-  const double *minv_u;            // 1 / lumped displacement mass
-  const double *rwp;               // sum over PML elements of rho_e * W_e at the node
-  const double *wps;               // sum over PML elements of W_e at the node
-};
-
-// Sparse per-stage source: entries grouped by stage tile.
-struct Source {
-  const int* tile_ptr;             // (ntiles + 1)
-  const long long* node;           // lattice node
-  const int* comp;                 // component 0..2
-  const double* m;                 // M^-1 of that DOF
-  const double *a, *a2;            // value_j = ca*((a[j] - a2[j]) * m[j]) + cb*((b[j] - b2[j]) * m[j])
-  const double *b, *b2;
-  double ca, cb;
-};
-
 struct StageArgs {
   Geo g;
-  NodeData nd;
   const double* in[6];             // stage: x1 x2 x3 g1 g2 g3; apply: in[0] = x
-  int stage;                       // 0 = apply (raw operator), 1..4 = RK4 stage
+  int stage;                       // 0 = apply (raw operator), 1..4 = RK4 stage, 5 = central difference
   int set_mask;                    // bit0: G on v1, bit1: K on v2, bit2: C on v3
   int transpose;                   // 1: apply the transposed operator
   double sgn[3];                   // multipliers on v1, v2, v3
-  double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed)
+  double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed); stage 5: dt^2, dt, dt^3
   double* out;                     // stage 1-3: g_s; apply: raw result
-  double* xnew[3];                 // stage 4: updated x1, x2, x3
-  Source src;
+  double* xnew[3];                 // stage 4: updated x1, x2, x3; stage 5: u_{n+1}, u_his
   int nsrc_tiles;                  // 0: no source
 };
 
@@ -134,7 +112,7 @@ struct Stage3Args {
   Geo3 g;
   const double* in[6];             // stage: x1 x2 x3 g1 g2 g3; apply: in[0] = x
   double* out;                     // stage 1-3: g_s; apply: raw result
-  double* xnew[3];                 // stage 4: updated x1, x2, x3
+  double* xnew[3];                 // stage 4: updated x1, x2, x3; stage 5: u_{n+1}, u_his
   const double* node;              // node table
   const double* tab;               // small table
   const Src3* src;                 // device pointer, used when src_mode != 0
@@ -145,7 +123,7 @@ struct Stage3Args {
   double ca, cb;                   // adjoint row weights
   int nshot;                       // shots in the launch; state vectors are shot-major
   long long shot_stride;           // elements per shot of a state vector (9 NN)
-  double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed)
+  double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed); stage 5: dt^2, dt, dt^3
 };
 
 // stage 0 = apply; transposed stages 1..4 carry the adjoint signs (-G, +K, -C)
