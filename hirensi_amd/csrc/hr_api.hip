@@ -301,6 +301,7 @@ Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
   B.g.tiles0 = g.tiles[0]; B.g.tiles1 = g.tiles[1]; B.g.tz = g.tz;
   for (size_t c = 0; c < ctx->zb.size() && c <= (size_t)HR_MAX_ZCHUNKS; ++c) B.g.zb[c] = ctx->zb[c];
   for (int a = 0; a < 3; ++a) B.g.dxi[a] = g.dxi[a];
+  B.g.detj = g.detj;
   for (int q = 0; q < 6; ++q) B.in[q] = A.in[q];
   B.out = A.out;
   for (int q = 0; q < 3; ++q) B.xnew[q] = A.xnew[q];
@@ -568,6 +569,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
   g.dxi[0] = (+j33 * j22 - 0.0 * 0.0) / detj;
   g.dxi[1] = (+j33 * j11 - 0.0 * 0.0) / detj;
   g.dxi[2] = (+j22 * j11 - 0.0 * 0.0) / detj;
+  g.detj = detj;
   const double w[3] = {0.33333333333333333333, 1.33333333333333333333, 0.33333333333333333333};
   for (int c = 0; c < 3; ++c)
     for (int b = 0; b < 3; ++b)

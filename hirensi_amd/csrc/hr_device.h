@@ -18,6 +18,7 @@ struct Geo {
   int tx, ty;                      // elements per tile in x and y
   int tz;                          // elements per z-chunk of a tile
   double dxi[3];                   // dxi/dx = DJI(k,k) of the affine element
+  double detj;                     // det J of the affine element
   double wq[27];                   // GLL weight * det J at local point a + 3b + 9c
   double pml[24];                  // PML_PARAM(6,4), column-major
 };
@@ -84,6 +85,7 @@ struct Geo3 {
   int top_rule;
   int tiles0, tiles1, tz;
   double dxi[3];
+  double detj;
   int zb[HR_MAX_ZCHUNKS + 1];      // z-chunk c spans element layers [zb[c], zb[c+1])
 };
 

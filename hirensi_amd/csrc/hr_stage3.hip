@@ -104,6 +104,40 @@ __device__ __forceinline__ bool rd3(const Geo3& g, int ex, int ey, int ez) {
 }
 __device__ __forceinline__ int sym3(int i, int k) { return i == k ? i : i + k + 2; }
 
+// 3-point GLL rule on xi = -1, 0, 1: weights and DM(a, c) = l_c'(xi_a)
+__device__ __forceinline__ constexpr double gll_w(int a) { return a == 1 ? 4.0 / 3.0 : 1.0 / 3.0; }
+__device__ __forceinline__ constexpr double gll_dm(int a, int c) {
+  return c == 0 ? (a - 1) - 0.5 : (c == 1 ? -2.0 * (a - 1) : (a - 1) + 0.5);
+}
+
+// One axis of the separable PML pull at lattice index i (nel elements along
+// the axis; node i is local point l_e of the one or two elements e that hold
+// it).  c[d] = sum_e w(a) DM(a, l_e) with a = l_e + d, d = -2..2 (index d + 2):
+// the coefficient of the value at i + d in the transposed derivative
+// sum_a DM(a, l) w(a) v(a).  By the summation-by-parts identity of the GLL rule
+// (w_a DM(a, b) + w_b DM(b, a) = 0 unless a = b = 0 or 2), the forward
+// derivative weighted by w(l_e), sum_e w(l_e) sum_c DM(l_e, c) v(c), has the
+// coefficients -c[d] for d != 0 and c[0] at d = 0.  m = sum_e w(l_e);
+// cov[d]: node i + d belongs to one of the elements.
+__device__ __forceinline__ void axis_stencil(int i, int nel, double c[5], double& m, bool cov[5]) {
+  m = 0.0;
+#pragma unroll
+  for (int d = 0; d < 5; ++d) { c[d] = 0.0; cov[d] = false; }
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    int l;
+    if (i & 1) { if (s) continue; l = 1; }
+    else if (s == 0) { if (i < 2) continue; l = 2; }
+    else { if ((i >> 1) >= nel) continue; l = 0; }
+    m += gll_w(l);
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+      const int a = l + d - 2;
+      if (a >= 0 && a <= 2) { c[d] += gll_w(a) * gll_dm(a, l); cov[d] = true; }
+    }
+  }
+}
+
 // Stage-S inputs (v1, v2, v3) of one field component from its raw inputs X
 // (x1 x2 x3 g1 g2 g3; apply: x) in the reference's nested VecWAXPY order.
 // NEG: adjoint signs on v1, v3.  S = 5 is one step of the central-difference
@@ -235,6 +269,33 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   const bool owned = owner && i < own_i1 && j < own_j1;
   const int rown = (ix + 2) + WX * (iy + 2);
   const bool oddx = i & 1, oddy = j & 1;
+
+  // PML-only tiles: every element around an owned node is a PML element, whose
+  // weights W(a, b, c) = det J w(a) w(b) w(c) factor per axis and whose inputs
+  // (nodal flux G, F-weighted w) are nodal.  The pull over the up to four x-y
+  // elements of a slab then separates: the x terms of the elements that share a
+  // y position are equal up to w(l_y), so they combine into one 5-point x
+  // stencil scaled by m_y = sum w(l_y), and likewise for y; the z terms of all
+  // of them are equal up to w(l_x) w(l_y).  Each owner keeps its stencil
+  // coefficients (scaled by det J dxi m) and LDS positions in registers;
+  // positions outside the elements read the node itself with coefficient 0.
+  double cxs[HAS_PML && !HAS_RD ? 5 : 1], cys[HAS_PML && !HAS_RD ? 5 : 1], azs = 0.0;
+  int pxs[HAS_PML && !HAS_RD ? 5 : 1], pys[HAS_PML && !HAS_RD ? 5 : 1];
+  if constexpr (HAS_PML && !HAS_RD) {
+    double cx[5], cy[5], mx, my;
+    bool vx[5], vy[5];
+    axis_stencil(i, g.nex, cx, mx, vx);
+    axis_stencil(j, g.ney, cy, my, vy);
+    const double fx = g.detj * g.dxi[0] * my, fy = g.detj * g.dxi[1] * mx;
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+      cxs[d] = fx * cx[d];
+      cys[d] = fy * cy[d];
+      pxs[d] = rown + (vx[d] ? d - 2 : 0);
+      pys[d] = rown + (vy[d] ? WX * (d - 2) : 0);
+    }
+    azs = g.detj * g.dxi[2] * mx * my;
+  }
 
   // owner registers: accumulators, diagonal terms, stage-4 carry
   // c3o: stage 4 x3 + dt/6 (g1 + 2 g2 + 2 g3); stage 5 2 u_n - u_{n-1}.  chs: stage 5 u_his
@@ -526,6 +587,47 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
       const double fz0 = sDM[3 * lz], fz1 = sDM[3 * lz + 1], fz2 = sDM[3 * lz + 2];
       const double tz0 = sDM[lz], tz1 = sDM[3 + lz], tz2 = sDM[6 + lz];
       double aE[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      if constexpr (HAS_PML && !HAS_RD) {
+        // separable pull (PML-only tiles): w(l_z) and the z stencil are uniform per slab
+        const double wl = gll_w(lz);
+        const double tzw0 = tz0 * gll_w(0), tzw1 = tz1 * gll_w(1), tzw2 = tz2 * gll_w(2);
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          const double* Gx = &sG[sP][3 * ii + 0][0];
+          const double* Gy = &sG[sP][3 * ii + 1][0];
+          double sx = 0.0, sy = 0.0;
+#pragma unroll
+          for (int d = 0; d < 5; ++d) {
+            sx += cxs[d] * Gx[pxs[d]];
+            sy += cys[d] * Gy[pys[d]];
+          }
+          const double sz =
+              ((0.0 + tzw0 * sG[z0][3 * ii + 2][rown]) + tzw1 * sG[z1][3 * ii + 2][rown]) + tzw2 * sG[z2][3 * ii + 2][rown];
+          acc[ii] += wl * (sx + sy) + azs * sz;
+        }
+        double dX[3], dY[3], dZ[3];
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          const double* Wx = &sW[sP][ii][0];
+          const double* Wy = &sW[sP][3 + ii][0];
+          double sx = cxs[2] * Wx[rown], sy = cys[2] * Wy[rown];
+#pragma unroll
+          for (int d = 0; d < 5; ++d) {
+            if (d == 2) continue;
+            sx -= cxs[d] * Wx[pxs[d]];
+            sy -= cys[d] * Wy[pys[d]];
+          }
+          dX[ii] = sx;
+          dY[ii] = sy;
+          dZ[ii] = azs * (((0.0 + fz0 * sW[z0][6 + ii][rown]) + fz1 * sW[z1][6 + ii][rown]) + fz2 * sW[z2][6 + ii][rown]);
+        }
+        aE[0] += wl * dX[0];
+        aE[1] += wl * dY[1];
+        aE[2] += wl * dZ[2];
+        aE[3] += wl * (dY[0] + dX[1]);
+        aE[4] += wl * (dZ[0] + dX[2]);
+        aE[5] += wl * (dZ[1] + dY[2]);
+      } else
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
