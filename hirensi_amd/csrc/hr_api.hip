@@ -119,9 +119,17 @@ struct hr_ctx {
   // stage-kernel timing
   int profiling = 0;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
-  long long prof_n[5] = {0, 0, 0, 0, 0};
-  double prof_ms[5] = {0, 0, 0, 0, 0};
+  // one launch in PROF_PERIOD is timed: (start, end) events around the kernel on
+  // each stream it uses; its span is the first start to the last end
+  struct StageEv {
+    int stage, n;
+    hipEvent_t start[3], end[3];
+  };
+  std::vector<StageEv> ev_open;
+  long long prof_tick = 0;
+  long long prof_all[5] = {0, 0, 0, 0, 0};    // launches while profiling
+  long long prof_n[5] = {0, 0, 0, 0, 0};      // timed launches
+  double prof_ms[5] = {0, 0, 0, 0, 0};        // their summed spans
 };
 
 namespace {
@@ -273,17 +281,28 @@ hipEvent_t take_event(hr_ctx* ctx) {
   return e;
 }
 
-// fold finished event pairs into the per-stage totals
+// timed launches: one in PROF_PERIOD, a period prime to the four RK4 stages
+constexpr long long PROF_PERIOD = 7;
+
+// fold finished event sets into the per-stage totals
 int drain_events(hr_ctx* ctx) {
   if (ctx->ev_open.empty()) return 0;
   HIPC(hipStreamSynchronize(ctx->stream));
   for (auto& p : ctx->ev_open) {
-    float ms = 0.0f;
-    HIPC(hipEventElapsedTime(&ms, p.second.first, p.second.second));
-    ctx->prof_n[p.first] += 1;
-    ctx->prof_ms[p.first] += ms;
-    ctx->ev_pool.push_back(p.second.first);
-    ctx->ev_pool.push_back(p.second.second);
+    float lo = 0.0f, hi = 0.0f;
+    for (int q = 0; q < p.n; ++q) {
+      float a = 0.0f, b = 0.0f;
+      HIPC(hipEventElapsedTime(&a, p.start[0], p.start[q]));
+      HIPC(hipEventElapsedTime(&b, p.start[0], p.end[q]));
+      lo = std::min(lo, a);
+      hi = std::max(hi, b);
+    }
+    ctx->prof_n[p.stage] += 1;
+    ctx->prof_ms[p.stage] += hi - lo;
+    for (int q = 0; q < p.n; ++q) {
+      ctx->ev_pool.push_back(p.start[q]);
+      ctx->ev_pool.push_back(p.end[q]);
+    }
   }
   ctx->ev_open.clear();
   return 0;
@@ -320,14 +339,18 @@ Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
 }
 
 int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  const bool prof = ctx->profiling && A.stage <= 4;   // per-kind statistics cover the apply and RK4 stages
+  // per-kind statistics cover the apply and RK4 stages
+  const bool counted = ctx->profiling && A.stage <= 4;
+  if (counted) ctx->prof_all[A.stage] += 1;
+  const bool prof = counted && (ctx->prof_tick++ % PROF_PERIOD) == 0;
+  hr_ctx::StageEv ev{A.stage, ctx->concurrent ? 3 : 1, {nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   if (prof) {
     if (ctx->ev_open.size() >= 4096) if (int rc = drain_events(ctx)) return rc;
-    e0 = take_event(ctx);
-    e1 = take_event(ctx);
-    if (!e0 || !e1) return fail(ctx, 3, "hipEventCreate failed");
-    HIPC(hipEventRecord(e0, ctx->stream));
+    for (int q = 0; q < ev.n; ++q) {
+      ev.start[q] = take_event(ctx);
+      ev.end[q] = take_event(ctx);
+      if (!ev.start[q] || !ev.end[q]) return fail(ctx, 3, "hipEventCreate failed");
+    }
   }
   {
     // 32-bit byte offsets: the node table (6 NN) and the 9 components must stay below 4 GiB
@@ -345,20 +368,23 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
       HIPC(hipEventRecord(ctx->ev_fork, ctx->stream));
       for (int q = 0; q < 2; ++q) {
         HIPC(hipStreamWaitEvent(ctx->cstream[q], ctx->ev_fork, 0));
+        if (prof) HIPC(hipEventRecord(ev.start[q + 1], ctx->cstream[q]));
         HIPC(launch_stage3(B, A.stage, A.transpose, q + 1, ctx->d_tl[q + 1], ctx->ntl[q + 1], ctx->cstream[q]));
+        if (prof) HIPC(hipEventRecord(ev.end[q + 1], ctx->cstream[q]));
         HIPC(hipEventRecord(ctx->ev_join[q], ctx->cstream[q]));
       }
+      if (prof) HIPC(hipEventRecord(ev.start[0], ctx->stream));
       HIPC(launch_stage3(B, A.stage, A.transpose, 0, ctx->d_tl[0], ctx->ntl[0], ctx->stream));
+      if (prof) HIPC(hipEventRecord(ev.end[0], ctx->stream));
       for (int q = 0; q < 2; ++q) HIPC(hipStreamWaitEvent(ctx->stream, ctx->ev_join[q], 0));
     } else {
+      if (prof) HIPC(hipEventRecord(ev.start[0], ctx->stream));
       for (int cls = 0; cls < 3; ++cls)
         HIPC(launch_stage3(B, A.stage, A.transpose, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
+      if (prof) HIPC(hipEventRecord(ev.end[0], ctx->stream));
     }
   }
-  if (prof) {
-    HIPC(hipEventRecord(e1, ctx->stream));
-    ctx->ev_open.push_back({A.stage, {e0, e1}});
-  }
+  if (prof) ctx->ev_open.push_back(ev);
   return 0;
 }
 
@@ -635,7 +661,8 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   F(ctx->d_img); F(ctx->d_restr); F(ctx->d_reg); F(ctx->d_cost_el); F(ctx->d_red); F(ctx->d_pm); F(ctx->d_pg);
   for (int c = 0; c < 3; ++c) F(ctx->d_tl[c]);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (auto& p : ctx->ev_open) { (void)hipEventDestroy(p.second.first); (void)hipEventDestroy(p.second.second); }
+  for (auto& p : ctx->ev_open)
+    for (int q = 0; q < p.n; ++q) { (void)hipEventDestroy(p.start[q]); (void)hipEventDestroy(p.end[q]); }
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   for (int q = 0; q < 2; ++q) {
@@ -1455,7 +1482,8 @@ int hr_set_profiling(hr_ctx* ctx, int mode) {
   if (!ctx) return 1;
   if (int rc = drain_events(ctx)) return rc;
   if (mode == 2) {
-    for (int s = 0; s < 5; ++s) { ctx->prof_n[s] = 0; ctx->prof_ms[s] = 0.0; }
+    for (int s = 0; s < 5; ++s) { ctx->prof_all[s] = ctx->prof_n[s] = 0; ctx->prof_ms[s] = 0.0; }
+    ctx->prof_tick = 0;
     return 0;
   }
   ctx->profiling = mode ? 1 : 0;
@@ -1465,7 +1493,10 @@ int hr_set_profiling(hr_ctx* ctx, int mode) {
 int hr_get_kernel_stats(hr_ctx* ctx, long long* launches, double* ms) {
   if (!ctx || !launches || !ms) return fail(ctx, 1, "null argument");
   if (int rc = drain_events(ctx)) return rc;
-  for (int s = 0; s < 5; ++s) { launches[s] = ctx->prof_n[s]; ms[s] = ctx->prof_ms[s]; }
+  for (int s = 0; s < 5; ++s) {
+    launches[s] = ctx->prof_all[s];
+    ms[s] = ctx->prof_n[s] ? ctx->prof_ms[s] * (double)ctx->prof_all[s] / (double)ctx->prof_n[s] : 0.0;
+  }
   return 0;
 }
 

@@ -185,10 +185,13 @@ double hr_recompute_ratio(const hr_ctx* ctx);
 #define HR_TILING_SERIAL_CLASSES 0x100
 int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version);
 
-/* Stage-kernel timing with HIP events recorded around every stage-kernel
- * launch on the context's stream (off by default; on = 1, reset = 2).
- * hr_get_kernel_stats fills launches[5] and ms[5] per stage kind
- * (0 = single-operator apply, 1..4 = RK4 stage). */
+/* Stage-kernel timing (off by default; on = 1, reset = 2).  One stage launch
+ * in seven is timed with HIP events recorded right before and after its
+ * kernel on each stream it uses (its span: first start to last end).
+ * hr_get_kernel_stats fills, per stage kind (0 = single-operator apply,
+ * 1..4 = RK4 stage), launches[5] = stage launches while profiling and
+ * ms[5] = their kernel time, the mean span of the timed launches times
+ * launches. */
 int hr_set_profiling(hr_ctx* ctx, int mode);
 int hr_get_kernel_stats(hr_ctx* ctx, long long* launches, double* ms);
 
