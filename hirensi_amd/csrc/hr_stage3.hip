@@ -513,7 +513,16 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
       else load(std::false_type{}, i, j, k, rown, k % RING, false);
     }
     {
-      const int nnew = 2 * ez + 3 - kfirst;
+      // Halo positions of a plane: rows 0, 1 of the window, then columns 0, 1 of
+      // the rows above.  The separable pull of PML-only tiles reads only the row
+      // and the column of an owned node, so there the four corner positions
+      // (rows 0, 1 x columns 0, 1) are skipped, and of the first slab of a chunk
+      // above another (a halo slab, whose only owners are those of its top plane)
+      // only the top plane's ring is loaded.
+      constexpr bool SEP = HAS_PML && !HAS_RD;
+      constexpr int HPL = SEP ? HP - 4 : HP;          // halo positions loaded per plane
+      const int pl0 = (SEP && halo) ? 2 : 0;          // first plane with its ring loaded
+      const int nnew = 2 * ez + 3 - kfirst - pl0;
       int hr_, nh;
       if (first) {
         // all three groups load a plane: wavefront 3 and the spare lanes take the halo
@@ -526,12 +535,17 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
         nh = 128;
       }
       if (hr_ >= 0) {
-        for (int hx = hr_; hx < nnew * HP; hx += nh) {
-          const int pl = hx / HP, q = hx - pl * HP;
+        for (int hx = hr_; hx < nnew * HPL; hx += nh) {
+          const int pl = hx / HPL, q = hx - pl * HPL;
           int lx_, ly_;
-          if (q < 2 * WX) { ly_ = q / WX; lx_ = q - ly_ * WX; }
-          else { const int q2 = q - 2 * WX; ly_ = 2 + (q2 >> 1); lx_ = q2 & 1; }
-          const int kk = kfirst + pl;
+          if constexpr (SEP) {
+            if (q < 2 * (WX - 2)) { ly_ = q / (WX - 2); lx_ = 2 + q - ly_ * (WX - 2); }
+            else { const int q2 = q - 2 * (WX - 2); ly_ = 2 + (q2 >> 1); lx_ = q2 & 1; }
+          } else {
+            if (q < 2 * WX) { ly_ = q / WX; lx_ = q - ly_ * WX; }
+            else { const int q2 = q - 2 * WX; ly_ = 2 + (q2 >> 1); lx_ = q2 & 1; }
+          }
+          const int kk = kfirst + pl0 + pl;
           load(std::false_type{}, gi0 + lx_, gj0 + ly_, kk, lx_ + WX * ly_, kk % RING, false);
         }
       }
