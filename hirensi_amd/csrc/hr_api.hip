@@ -134,14 +134,18 @@ struct hr_ctx {
 
 namespace {
 
-size_t field_size(const hr_ctx* ctx) { return (size_t)NCOMP * ctx->g.NN; }
+// State vectors: displacement block [3 n + c], then the stress block from
+// stress_base = 3 NN rounded up to an even count [stress_base + 6 n + (c - 3)]: a
+// node's components are contiguous and its six stress components 16-byte aligned
+// (the field size 9 NN + (NN odd) is even, so every shot's vectors are too).
+long long stress_base(const Geo& g) { return 3 * g.NN + (g.NN & 1); }
+size_t field_size(const hr_ctx* ctx) { return (size_t)(stress_base(ctx->g) + 6 * ctx->g.NN); }
 
-// State-vector slot of component c of lattice node n: displacement block [3 n + c],
-// then stress block [3 NN + 6 n + (c - 3)] (a node's components are contiguous).
-long long slot_of(const Geo& g, int c, long long n) { return c < 3 ? 3 * n + c : 3 * g.NN + 6 * n + (c - 3); }
+// State-vector slot of component c of lattice node n
+long long slot_of(const Geo& g, int c, long long n) { return c < 3 ? 3 * n + c : stress_base(g) + 6 * n + (c - 3); }
 void slot_decode(const Geo& g, long long slot, int& c, long long& n) {
   if (slot < 3 * g.NN) { n = slot / 3; c = (int)(slot - 3 * n); }
-  else { const long long r = slot - 3 * g.NN; n = r / 6; c = 3 + (int)(r - 6 * n); }
+  else { const long long r = slot - stress_base(g); n = r / 6; c = 3 + (int)(r - 6 * n); }
 }
 
 int fail(hr_ctx* c, int code, const char* fmt, ...) {
@@ -354,7 +358,7 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
   }
   {
     // 32-bit byte offsets: the node table (6 NN) and the 9 components must stay below 4 GiB
-    if ((unsigned long long)ctx->g.NN * 8ull * 9ull >= (1ull << 32))
+    if ((unsigned long long)field_size(ctx) * 8ull >= (1ull << 32))
       return fail(ctx, 3, "lattice of %lld nodes is too large for the v3 stage kernel", ctx->g.NN);
     // the v3 kernel fixes the adjoint signs by the transpose flag of stages 1-4
     const bool adj_signs = A.sgn[0] == -1.0 && A.sgn[1] == 1.0 && A.sgn[2] == -1.0;
@@ -631,7 +635,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
   if (int rc = upload(ctx, &ctx->d_xc, xc)) return rc;
   if (int rc = upload(ctx, &ctx->d_yc, yc)) return rc;
   if (int rc = upload(ctx, &ctx->d_zc, zc)) return rc;
-  const size_t nf = (size_t)NCOMP * g.NN;
+  const size_t nf = field_size(ctx);
   for (int a = 0; a < 3; ++a) {
     if (int rc = dalloc(ctx, &ctx->x[a], nf)) return rc;
     if (int rc = dalloc(ctx, &ctx->y[a], nf)) return rc;
@@ -686,7 +690,7 @@ int hr_get_store_eq(hr_ctx* ctx, long long* store_eq) {
   if (!ctx || !store_eq) return fail(ctx, 1, "null argument");
   // inverse numbering for the box
   const Geo& g = ctx->g;
-  std::vector<long long> inv((size_t)NCOMP * g.NN, -1);
+  std::vector<long long> inv(field_size(ctx), -1);
   for (long long e = 0; e < ctx->neq; ++e) inv[ctx->eq_slot[e]] = e;
   long long t = 0;
   for (int c = 0; c < 3; ++c)
@@ -1325,8 +1329,8 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     HIPC(hipMemcpyAsync(g_lambda, dgl, NN * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(hipMemcpyAsync(g_mu, dgm, NN * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
-    HIPC(hipMemsetAsync(ctx->tmp, 0, NN * sizeof(double) * NCOMP, ctx->stream));
-    HIPC(hipMemsetAsync(ctx->tmp2, 0, NN * sizeof(double) * NCOMP, ctx->stream));
+    HIPC(hipMemsetAsync(ctx->tmp, 0, field_size(ctx) * sizeof(double), ctx->stream));
+    HIPC(hipMemsetAsync(ctx->tmp2, 0, field_size(ctx) * sizeof(double), ctx->stream));
   }
   if (misfit) {
     if (!ctx->d_scalar) if (int rc = dalloc(ctx, &ctx->d_scalar, 1)) return rc;

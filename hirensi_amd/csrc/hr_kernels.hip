@@ -3,10 +3,11 @@
 // regular-domain history slices, the per-step lambda/mu gradient, misfit.
 //
 // State layout: per state vector, the displacement block [3 * n + c]
-// (c = u_x, u_y, u_z) followed by the stress block [3 NN + 6 * n + m]
-// (m = s_xx s_yy s_zz s_xy s_xz s_yz), n = i + NX * (j + NY * k): the
-// components of a node are contiguous.  Inactive entries are zero and never
-// written.
+// (c = u_x, u_y, u_z) followed by the stress block [SB + 6 * n + m]
+// (m = s_xx s_yy s_zz s_xy s_xz s_yz; SB = 3 NN rounded up to an even count,
+// so that a node's stress components are 16-byte aligned),
+// n = i + NX * (j + NY * k): the components of a node are contiguous.
+// Inactive entries are zero and never written.
 
 #include <hip/hip_runtime.h>
 #include "hr_device.h"

@@ -25,7 +25,9 @@
 //    stage 4 x3 and the partial g-sum) in registers, and writes the x1, x2
 //    updates of stage 4 at load time; helpers load the low-side halo;
 //  * state vectors keep a node's components contiguous (displacement block
-//    [3 n + c], then stress block [3 NN + 6 n + m]), so a window row touches
+//    [3 n + c], then stress block [SB + 6 n + m], SB = 3 NN rounded up to
+//    an even count: a node's six stress components are three aligned 16-byte
+//    accesses), so a window row touches
 //    about half the cache lines of a component-major layout; 32-bit byte
 //    offsets on uniform bases; small constant tables (GLL derivative matrix,
 //    weights) in LDS.
@@ -72,6 +74,12 @@ __device__ __forceinline__ void st3(double* p, unsigned off, double a, double b,
   t.v[1] = b;
   t.v[2] = c;
   *reinterpret_cast<D3*>(reinterpret_cast<char*>(p) + off) = t;
+}
+// six consecutive components at a 16-byte aligned offset: three 16-byte stores
+__device__ __forceinline__ void st6(double* p, unsigned off, const double* v) {
+  char* q = reinterpret_cast<char*>(p) + off;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) *reinterpret_cast<double2*>(q + 16 * a) = make_double2(v[2 * a], v[2 * a + 1]);
 }
 // stores of three consecutive components, each only where its bit of act is set
 __device__ __forceinline__ void st3m(double* p, unsigned off, int act, double a, double b, double c) {
@@ -248,7 +256,9 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   const int mask = (S == 0) ? A.set_mask : 7;
   const bool useG = mask & 1, useK = mask & 2, useC = mask & 4;
   const unsigned NN8 = (unsigned)g.NN * 8u;   // bytes of one double per lattice node
-  const unsigned SB = 3u * NN8;                 // byte offset of the stress block of a state vector
+  // byte offset of the stress block of a state vector: 3 NN doubles rounded up to an
+  // even count, so that a node's six stress components are 16-byte aligned
+  const unsigned SB = 8u * (3u * (unsigned)g.NN + ((unsigned)g.NN & 1u));
   const double* __restrict__ tab = A.tab;
   const double* __restrict__ nodet = A.node;
 
@@ -422,12 +432,18 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
           }
           const double l2m = lam + 2.0 * mu, m2 = 2.0 * mu;
           const int sm = smask3(g, ii, jj, kk);
-          // ---- stress components in two halves (normal, shear): G splits the same way
+          // ---- stress components in two halves (normal, shear): G splits the same way.
+          // The node's six stress components of each input: three 16-byte loads
+          double XS6[6][NIN];
+#pragma unroll
+          for (int q = 0; q < NIN; ++q) {
+            ld2(inp[q], offs, XS6[0][q], XS6[1][q]);
+            ld2(inp[q], offs + 16u, XS6[2][q], XS6[3][q]);
+            ld2(inp[q], offs + 32u, XS6[4][q], XS6[5][q]);
+          }
 #pragma unroll
           for (int hf = 0; hf < 2; ++hf) {
-            double XS[3][NIN];
-#pragma unroll
-            for (int q = 0; q < NIN; ++q) ld3(inp[q], offs + 24u * hf, XS[0][q], XS[1][q], XS[2][q]);
+            const double(*XS)[NIN] = &XS6[3 * hf];
             double s1[3], s2[3], s3[3];
 #pragma unroll
             for (int t = 0; t < 3; ++t) combine<S, NEG>(XS[t], A.hh, A.h, A.h6, mask, s1[t], s2[t], s3[t]);
@@ -789,14 +805,20 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
         double* const dst = (S == 4) ? xnp[2] : (S == 5 ? xnp[0] : outp);
         if (ua) st3(dst, offu, out[0], out[1], out[2]);
         if constexpr (HAS_PML) {
-          if (sm & 7) st3m(dst, offs, sm & 7, out[3], out[4], out[5]);
-          if (sm & 56) st3m(dst, offs + 24u, (sm >> 3) & 7, out[6], out[7], out[8]);
+          if (sm == 0x3f) st6(dst, offs, &out[3]);
+          else {
+            if (sm & 7) st3m(dst, offs, sm & 7, out[3], out[4], out[5]);
+            if (sm & 56) st3m(dst, offs + 24u, (sm >> 3) & 7, out[6], out[7], out[8]);
+          }
         }
         if constexpr (S == 5) {
           if (ua) st3(xnp[1], offu, his[0], his[1], his[2]);
           if constexpr (HAS_PML) {
-            if (sm & 7) st3m(xnp[1], offs, sm & 7, his[3], his[4], his[5]);
-            if (sm & 56) st3m(xnp[1], offs + 24u, (sm >> 3) & 7, his[6], his[7], his[8]);
+            if (sm == 0x3f) st6(xnp[1], offs, &his[3]);
+            else {
+              if (sm & 7) st3m(xnp[1], offs, sm & 7, his[3], his[4], his[5]);
+              if (sm & 56) st3m(xnp[1], offs + 24u, (sm >> 3) & 7, his[6], his[7], his[8]);
+            }
           }
         }
       }
