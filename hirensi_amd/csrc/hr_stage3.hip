@@ -307,6 +307,15 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
     azs = g.detj * g.dxi[2] * mx * my;
   }
 
+  // PML stretch factors of the owner column (x and y depend on i and j only)
+  double col_ax = 0.0, col_bx = 0.0, col_ay = 0.0, col_by = 0.0;
+  if constexpr (HAS_PML) {
+    if (owner && i < g.NX && j < g.NY) {
+      ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * i), col_ax, col_bx);
+      ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * j), col_ay, col_by);
+    }
+  }
+
   // owner registers: accumulators, diagonal terms, stage-4 carry
   // c3o: stage 4 x3 + dt/6 (g1 + 2 g2 + 2 g3); stage 5 2 u_n - u_{n-1}.  chs: stage 5 u_his
   // acc starts at the node's diagonal terms (set when the owner loads its node)
@@ -324,8 +333,11 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   // in one batch: inactive lattice entries are zero in every state vector and
   // never written, and contribute nothing.  Activity only gates the stores.
   constexpr int NIN = (S == 0) ? 1 : (S == 5 ? 3 : 2 + S);   // x1 x2 x3 g1 g2 g3 prefix (apply: x)
-  auto load = [&](auto own_tag, int ii, int jj, int kk, int r, int slot, bool own_plane) {
+  // COL: the node is in the lane's owner column (ii, jj = i, j) and its plane kk is
+  // the same for the whole wavefront
+  auto load = [&](auto own_tag, auto col_tag, int ii, int jj, int kk, int r, int slot, bool own_plane) {
     constexpr bool OWN = decltype(own_tag)::value;
+    constexpr bool COL = decltype(col_tag)::value;
     constexpr bool ALL_U = HAS_PML || (OWN && UPD);      // v1 and v3 of u, or the state update
     if constexpr (OWN) {
       // a newly loaded owned node starts its sums at its diagonal terms (zero off the PML)
@@ -383,13 +395,21 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
         // owner registers set here are read for PML nodes only.
         if (HAS_RD ? pn : true) {
           // ---- stretch factors, material
-          const unsigned ox = 8u * (unsigned)(TAB_STRETCH + 2 * ii);
-          const unsigned oy = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * jj);
-          const unsigned oz = 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * g.NY + 2 * kk);
           double ax, bx, ay, by, az, bz;
-          ld2(tab, ox, ax, bx);
-          ld2(tab, oy, ay, by);
-          ld2(tab, oz, az, bz);
+          if constexpr (COL) {
+            // x, y factors of the owner column from registers; z factors of the
+            // wavefront's plane with one uniform load
+            ax = col_ax; bx = col_bx; ay = col_ay; by = col_by;
+            const int ku = __builtin_amdgcn_readfirstlane(kk);
+            const __attribute__((address_space(4))) double* tz_ =
+                (const __attribute__((address_space(4))) double*)tab + (TAB_STRETCH + 2 * g.NX + 2 * g.NY + 2 * ku);
+            az = tz_[0];
+            bz = tz_[1];
+          } else {
+            ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * ii), ax, bx);
+            ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * jj), ay, by);
+            ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * g.NY + 2 * kk), az, bz);
+          }
           double lam, mu;
           if constexpr (HAS_RD) { lam = lmv[0]; mu = lmv[1]; }
           else ld2(nodet, offp, lam, mu);
@@ -525,8 +545,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
     // ------------------------------------------------------------ loads
     if (owner && k >= kfirst) {
       const bool own_plane = k >= 2 * ez0 && (k < 2 * ez_end || k == g.NZ - 1);
-      if (owned) load(std::true_type{}, i, j, k, rown, k % RING, own_plane);
-      else load(std::false_type{}, i, j, k, rown, k % RING, false);
+      if (owned) load(std::true_type{}, std::true_type{}, i, j, k, rown, k % RING, own_plane);
+      else load(std::false_type{}, std::true_type{}, i, j, k, rown, k % RING, false);
     }
     {
       // Halo positions of a plane: rows 0, 1 of the window, then columns 0, 1 of
@@ -562,7 +582,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
             else { const int q2 = q - 2 * WX; ly_ = 2 + (q2 >> 1); lx_ = q2 & 1; }
           }
           const int kk = kfirst + pl0 + pl;
-          load(std::false_type{}, gi0 + lx_, gj0 + ly_, kk, lx_ + WX * ly_, kk % RING, false);
+          load(std::false_type{}, std::false_type{}, gi0 + lx_, gj0 + ly_, kk, lx_ + WX * ly_, kk % RING, false);
         }
       }
     }
