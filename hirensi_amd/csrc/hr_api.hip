@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <functional>
 #include "hr_device.h"
 #include "../../include/hirensi.h"
 
@@ -111,6 +112,11 @@ struct hr_ctx {
   hipStream_t cstream[2] = {nullptr, nullptr};
   hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   int concurrent = 1;
+  // Per-step work (receiver traces, history slices, the per-step gradient) that
+  // only reads the state a stage-1 launch reads and does not write: queued on the
+  // main stream after the next stage launch's fork, so that it runs beside the
+  // side classes instead of between stages.  The next launch_stage consumes it.
+  std::vector<std::function<int()>> deferred;
   // stage-kernel version and tile-class lists
   int version = DEFAULT_VERSION;
   std::vector<int> zb;                      // z-chunk boundaries (element layers)
@@ -342,6 +348,15 @@ Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
   return B;
 }
 
+// queue the deferred per-step work on the main stream (in the order it was deferred)
+int run_deferred(hr_ctx* ctx) {
+  std::vector<std::function<int()>> work;
+  work.swap(ctx->deferred);
+  for (auto& f : work)
+    if (int rc = f()) return rc;
+  return 0;
+}
+
 int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
   // per-kind statistics cover the apply and RK4 stages
   const bool counted = ctx->profiling && A.stage <= 4;
@@ -380,8 +395,10 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
       if (prof) HIPC(hipEventRecord(ev.start[0], ctx->stream));
       HIPC(launch_stage3(B, A.stage, A.transpose, 0, ctx->d_tl[0], ctx->ntl[0], ctx->stream));
       if (prof) HIPC(hipEventRecord(ev.end[0], ctx->stream));
+      if (int rc = run_deferred(ctx)) return rc;
       for (int q = 0; q < 2; ++q) HIPC(hipStreamWaitEvent(ctx->stream, ctx->ev_join[q], 0));
     } else {
+      if (int rc = run_deferred(ctx)) return rc;
       if (prof) HIPC(hipEventRecord(ev.start[0], ctx->stream));
       for (int cls = 0; cls < 3; ++cls)
         HIPC(launch_stage3(B, A.stage, A.transpose, cls, ctx->d_tl[cls], ctx->ntl[cls], ctx->stream));
@@ -531,7 +548,10 @@ int run_forward_steps(hr_ctx* ctx, int n0, int n1, double* seg_hist, size_t seg_
   // steps n0 .. n1-1 from the current state; stores slices n0..n1 into seg_hist when given
   const double sgn[3] = {1.0, 1.0, 1.0};
   for (int n = n0; n < n1; ++n) {
-    if (seg_hist) if (int rc = store_slice(ctx, seg_hist, seg_rows, (size_t)(n - n0))) return rc;
+    // the slice of the state at the start of step n, queued beside its stage 1
+    if (seg_hist) ctx->deferred.push_back([ctx, seg_hist, seg_rows, n, n0]() {
+      return store_slice(ctx, seg_hist, seg_rows, (size_t)(n - n0));
+    });
     Src3Spec sp4[4];
     forward_sources(ctx, n, sp4);
     if (int rc = rk4_step(ctx, ctx->f_dt, +1, 0, sgn, sp4)) return rc;
@@ -938,6 +958,7 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
                  const long long* src_eq, const double* src_val, int nrec, const long long* rec_eq, double* traces,
                  int history, int segment, int scheme) {
   if (int rc = check_ready(ctx)) return rc;
+  ctx->deferred.clear();   // work left by a solve that stopped on an error
   if (nshot < 1) return fail(ctx, 1, "nshot must be >= 1");
   if (nstep < 1 || !(dt > 0.0)) return fail(ctx, 1, "nstep must be >= 1 and dt > 0");
   if (!ftab) return fail(ctx, 1, "null time-function table");
@@ -1062,8 +1083,10 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
   const double sgn[3] = {1.0, 1.0, 1.0};
   const size_t nfs = field_size(ctx) * nshot;
   for (int n = 0; n < nstep; ++n) {
-    if (int rc = record_step(ctx, n)) return rc;
-    if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist, rows, (size_t)n)) return rc;
+    // traces and history slice of the state at the start of step n, queued beside its stage 1
+    ctx->deferred.push_back([ctx, n]() { return record_step(ctx, n); });
+    if (history == HR_HISTORY_STORE)
+      ctx->deferred.push_back([ctx, rows, n]() { return store_slice(ctx, ctx->d_hist, rows, (size_t)n); });
     if (history == HR_HISTORY_CHECKPOINT && n % segment == 0) {
       double* cp = nullptr;
       if (int rc = dalloc(ctx, &cp, 3 * nfs)) return rc;
@@ -1162,6 +1185,7 @@ int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_mea
 int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long long* rec_eq, const double* dis_meas,
                         double* g_lambda, double* g_mu, double* misfit) {
   if (int rc = check_ready(ctx)) return rc;
+  ctx->deferred.clear();   // work left by a solve that stopped on an error
   if (nstep != ctx->f_nstep || nrec != ctx->f_nrec || dt != ctx->f_dt)
     return fail(ctx, 1, "nstep/dt/nrec differ from the last hr_forward");
   if (ctx->f_history == HR_HISTORY_NONE) return fail(ctx, 2, "the last hr_forward kept no history");
@@ -1284,9 +1308,14 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
         uh_rows = seg_rows;
         uh_row = (size_t)(n - seg_lo);
       }
-      for (int q = 0; q < nshot; ++q)
-        HIPC(launch_gradient(ctx->g, ctx->box0, ctx->boxn, uh + ((size_t)q * uh_rows + uh_row) * slice, adj[1] + q * nf,
-                             dt, ctx->d_gl, ctx->d_gm, ctx->stream));
+      // queued beside stage 1 of the adjoint step, which reads but does not write adj
+      double* const a1 = adj[1];
+      ctx->deferred.push_back([ctx, nshot, uh, uh_rows, uh_row, slice, a1, nf, dt]() -> int {
+        for (int q = 0; q < nshot; ++q)
+          HIPC(launch_gradient(ctx->g, ctx->box0, ctx->boxn, uh + ((size_t)q * uh_rows + uh_row) * slice,
+                               a1 + q * nf, dt, ctx->d_gl, ctx->d_gm, ctx->stream));
+        return 0;
+      });
     }
     // misfit sources at t_n (stage 1), t_{n-1} (stage 4), mean (stages 2, 3)
     Src3Spec sp4[4];
