@@ -29,7 +29,7 @@ struct HipError {
 
 // per-launch source parameters of the v3 stage kernel
 struct Src3Spec {
-  const Src3* dev = nullptr;
+  const Src3* lists = nullptr;   // host copy of the solve's source lists
   int mode = 0, ra = 0, rb = -1, step = 0, fcol = 0;
   double ca = 0.0, cb = 0.0;
 };
@@ -56,8 +56,7 @@ struct hr_ctx {
   double *d_xc = nullptr, *d_yc = nullptr, *d_zc = nullptr;
   double* d_node = nullptr;                 // v3 node table (NT_COUNT x NN)
   double* d_tab = nullptr;                  // v3 small table (weights, per-axis stretch)
-  Src3* d_src3_fwd = nullptr;               // v3 source lists (device structs)
-  Src3* d_src3_adj = nullptr;
+  Src3 src3_fwd{}, src3_adj{};              // source lists of the last forward / adjoint (device arrays)
   std::vector<double> h_minv_u, h_wps, h_a;  // host copies for M^-1 of sources, diag M
   bool have_material = false, have_pml = false;
   // state (device, 9*NN each)
@@ -339,7 +338,7 @@ Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
   B.set_mask = A.set_mask;
   B.hh = A.hh; B.h = A.h; B.h6 = A.h6;
   if (sp && sp->mode) {
-    B.src = sp->dev; B.src_mode = sp->mode; B.ra = sp->ra; B.rb = sp->rb; B.ca = sp->ca; B.cb = sp->cb;
+    B.src = *sp->lists; B.src_mode = sp->mode; B.ra = sp->ra; B.rb = sp->rb; B.ca = sp->ca; B.cb = sp->cb;
     B.step = sp->step; B.fcol = sp->fcol;
   }
   // stage launches over the state buffers cover every shot; hr_apply works on one field
@@ -537,7 +536,7 @@ void forward_sources(hr_ctx* ctx, int n, Src3Spec sp4[4]) {
   for (int s = 0; s < 4; ++s) {
     sp4[s] = Src3Spec();
     if (!ctx->d_src_tile_ptr) continue;
-    sp4[s].dev = ctx->d_src3_fwd;
+    sp4[s].lists = &ctx->src3_fwd;
     sp4[s].mode = 1;
     sp4[s].step = n;
     sp4[s].fcol = (s == 0) ? 0 : (s == 3 ? 2 : 1);
@@ -674,7 +673,7 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   if (!ctx) return 0;
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   F(ctx->d_eq_slot); F(ctx->d_xc); F(ctx->d_yc); F(ctx->d_zc);
-  F(ctx->d_node); F(ctx->d_tab); F(ctx->d_src3_fwd); F(ctx->d_src3_adj); F(ctx->d_ftab);
+  F(ctx->d_node); F(ctx->d_tab); F(ctx->d_ftab);
   for (int a = 0; a < 3; ++a) { F(ctx->x[a]); F(ctx->y[a]); F(ctx->gs[a]); F(ctx->shell[a].l); F(ctx->shell[a].r); }
   F(ctx->tmp); F(ctx->tmp2); F(ctx->d_traces); F(ctx->d_rec_slot); F(ctx->d_hist); F(ctx->d_seg);
   for (double* p : ctx->ckpt) F(p);
@@ -936,7 +935,7 @@ int cd_step(hr_ctx* ctx, double dt, int n) {
   Src3Spec sp;
   if (ctx->d_src_tile_ptr) {
     A.nsrc_tiles = 1;
-    sp.dev = ctx->d_src3_fwd;
+    sp.lists = &ctx->src3_fwd;
     sp.mode = 1;
     sp.step = n;
     sp.fcol = 0;
@@ -1037,7 +1036,7 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
     std::memset(&h3, 0, sizeof(h3));
     h3.tile_ptr = ctx->d_src_tile_ptr; h3.node = ctx->d_src_node; h3.comp = ctx->d_src_comp;
     h3.m = ctx->d_src_m; h3.val = ctx->d_src_val; h3.ftab = ctx->d_ftab; h3.nstep = nstep; h3.ntiles = nt;
-    if (int rc = upload(ctx, &ctx->d_src3_fwd, std::vector<Src3>{h3})) return rc;
+    ctx->src3_fwd = h3;
   } else if (ctx->d_src_tile_ptr) {
     (void)hipFree(ctx->d_src_tile_ptr);
     ctx->d_src_tile_ptr = nullptr;
@@ -1241,7 +1240,7 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     h3.tile_ptr = ctx->d_rec_tile_ptr; h3.node = ctx->d_rec_node; h3.comp = ctx->d_rec_comp;
     h3.m = ctx->d_rec_m; h3.tr = ctx->d_tr_sorted; h3.obs = ctx->d_obs_sorted; h3.nrec = nrec;
     h3.nstep = nstep; h3.ntiles = ntiles(ctx->g);
-    if (int rc = upload(ctx, &ctx->d_src3_adj, std::vector<Src3>{h3})) return rc;
+    ctx->src3_adj = h3;
   }
   if (nrec) {
     HIPC(launch_permute_cols((long long)nshot * rows, nrec, ctx->d_rec_perm, ctx->d_traces, ctx->d_tr_sorted,
@@ -1322,7 +1321,7 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     for (int s = 0; s < 4; ++s) {
       sp4[s] = Src3Spec();
       if (!nrec) continue;
-      sp4[s].dev = ctx->d_src3_adj;
+      sp4[s].lists = &ctx->src3_adj;
       sp4[s].mode = 2;
       if (s == 0) { sp4[s].ra = n; sp4[s].ca = 1.0; }
       else if (s == 3) { sp4[s].ra = n - 1; sp4[s].ca = 1.0; }

@@ -248,6 +248,14 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
   const int ty = (b / g.tiles0) % g.tiles1;
   const int tz = b / (g.tiles0 * g.tiles1);
   const int ex0 = tx * TX, ey0 = ty * TY, ez0 = g.zb[tz];
+  // this (shot, tile)'s source entries [src_j0, src_j1), read once per block (uniform)
+  int src_j0 = 0, src_j1 = 0;
+  if (A.src_mode) {
+    const __attribute__((address_space(4))) int* tptr = (const __attribute__((address_space(4))) int*)A.src.tile_ptr;
+    const int tp = shot * A.src.ntiles + b;
+    src_j0 = tptr[tp];
+    src_j1 = tptr[tp + 1];
+  }
   const int ez_end = g.zb[tz + 1];
   const int gi0 = 2 * ex0 - 2, gj0 = 2 * ey0 - 2;
   const int own_i1 = (ex0 + TX >= g.nex) ? g.NX : 2 * (ex0 + TX);
@@ -783,11 +791,9 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S
 #pragma unroll
           for (int m = 0; m < 9; ++m) {
             double src = 0.0;
-            if (m < 3 && A.src_mode && ua) {
-              const Src3* sp = A.src;
-              const int tp = shot * sp->ntiles + b;
-              const int j0 = sp->tile_ptr[tp], j1 = sp->tile_ptr[tp + 1];
-              for (int jj = j0; jj < j1; ++jj) {
+            if (m < 3 && src_j1 > src_j0 && ua) {
+              const Src3* sp = &A.src;
+              for (int jj = src_j0; jj < src_j1; ++jj) {
                 if (sp->node[jj] == (long long)n && sp->comp[jj] == m) {
                   const double mm = sp->m[jj];
                   double val;
