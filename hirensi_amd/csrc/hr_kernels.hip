@@ -52,12 +52,13 @@ __global__ void record_kernel(int nrec, const long long* __restrict__ slot, cons
 // displacement of the regular-domain node box -> history slice (3 x nbox)
 __global__ void store_box_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int bny, int bnz,
                                  const double* __restrict__ x2, double* __restrict__ hist) {
-  const long long nbox = (long long)bnx * bny * bnz;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < 3 * nbox; t += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(t / nbox);
-    const long long r = t - c * nbox;
-    const int bi = (int)(r % bnx), bj = (int)((r / bnx) % bny), bk = (int)(r / ((long long)bnx * bny));
-    const long long n = (long long)(bi0 + bi) + (long long)g.NX * ((long long)(bj0 + bj) + (long long)g.NY * (bk0 + bk));
+  // 32-bit indices: 9 NN < 2^31 (checked by the host)
+  const int nbox = bnx * bny * bnz;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 3 * nbox; t += gridDim.x * blockDim.x) {
+    const int c = t / nbox;
+    const int r = t - c * nbox;
+    const int bi = r % bnx, bj = (r / bnx) % bny, bk = r / (bnx * bny);
+    const int n = (bi0 + bi) + g.NX * ((bj0 + bj) + g.NY * (bk0 + bk));
     hist[t] = x2[3 * n + c];
   }
 }
@@ -70,13 +71,14 @@ __global__ void store_box_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int 
 __global__ void gradient_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int bny, int bnz,
                                 const double* __restrict__ uh, const double* __restrict__ p2, double dt,
                                 double* __restrict__ glam, double* __restrict__ gmu) {
-  const long long nbox = (long long)bnx * bny * bnz;
-  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nbox; r += (long long)gridDim.x * blockDim.x) {
-    const int bi = (int)(r % bnx), bj = (int)((r / bnx) % bny), bk = (int)(r / ((long long)bnx * bny));
+  // 32-bit indices: 9 NN < 2^31 (checked by the host)
+  const int nbox = bnx * bny * bnz;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nbox; r += gridDim.x * blockDim.x) {
+    const int bi = r % bnx, bj = (r / bnx) % bny, bk = r / (bnx * bny);
     const int pos[3] = {bi0 + bi, bj0 + bj, bk0 + bk};
-    const long long n = (long long)pos[0] + (long long)g.NX * ((long long)pos[1] + (long long)g.NY * pos[2]);
-    const long long stride[3] = {1, bnx, (long long)bnx * bny};
-    const long long lstride[3] = {1, g.NX, (long long)g.NX * g.NY};
+    const int n = pos[0] + g.NX * (pos[1] + g.NY * pos[2]);
+    const int stride[3] = {1, bnx, bnx * bny};
+    const int lstride[3] = {1, g.NX, g.NX * g.NY};
     // per axis two element sides: s = 0 the element below (local 2; local 1 for an
     // odd node), s = 1 the element above (local 0).  Only sides whose element is
     // in the regular-domain range of the axis are used: their stencils stay in
@@ -103,7 +105,7 @@ __global__ void gradient_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int b
           if (has[a][s]) {
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
-              const long long off = (long long)(m - l);
+              const int off = m - l;
               su += DM(l, m) * uh[c * nbox + r + off * stride[a]];
               sp += DM(l, m) * p2[3 * (n + off * lstride[a]) + c];
             }
