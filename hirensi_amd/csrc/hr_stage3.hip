@@ -199,11 +199,14 @@ struct Shape3 {
   static constexpr int HP = WP - OC;                                      // halo positions per plane
 };
 
-// Occupancy hints: the LDS of the PML-only class allows two blocks per CU, but
-// asking for three at stages 1-3 gives a register allocation that measured 2 %
-// faster (the compiler reports the unmet target).
+// Occupancy hints: LDS allows two blocks per CU of the mixed and PML-only
+// classes and four of the regular-domain class, but asking for one more (the
+// mixed and regular-domain classes; the PML-only class at stages 1-3) gives
+// register allocations that measured 6 %, 1.5 % and 2 % faster in the whole
+// gradient (the compiler reports the unmet targets).  At stage 4 the PML-only
+// class would then need more than 256 registers.
 template <int TX, int TY, bool HAS_RD, bool HAS_PML, int S, bool TR>
-__global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 2 : 4) : (S >= 1 && S <= 3 ? 3 : 2)) stage_kernel3(const Stage3Args A, const int* __restrict__ tiles, int ntl) {
+__global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S <= 3 ? 3 : 2)) stage_kernel3(const Stage3Args A, const int* __restrict__ tiles, int ntl) {
   using SH = Shape3<TX, TY>;
   constexpr int WX = SH::WX, WP = SH::WP, NEW = SH::NEW, OX = SH::OX, OC = SH::OC, HP = SH::HP;
   static_assert(OC <= 64 && NT3 == 256, "one owner plane group per wavefront, plus one helper wavefront");
