@@ -19,7 +19,7 @@
 using namespace hr;
 
 namespace {
-constexpr int DEFAULT_TZ = 6;   // element layers per z-chunk (sweep of 3..8 at 64x64x32: 6 first)
+constexpr int DEFAULT_TZ = 7;   // element layers per z-chunk (sweep of 3..8 at 64x64x32: 7 first)
 constexpr int DEFAULT_TY = 3;
 constexpr int DEFAULT_VERSION = 3;
 
