@@ -10,7 +10,9 @@
 #include <vector>
 #include <algorithm>
 #include <functional>
+#include <unordered_map>
 #include "hr_device.h"
+#include "hr_revolve.h"
 #include "../../include/hirensi.h"
 
 #pragma clang fp contract(off)
@@ -79,10 +81,27 @@ struct hr_ctx {
   long long* d_rec_slot = nullptr;
   double* d_hist = nullptr;                 // store mode: (nstep+1) x 3 x nbox
   size_t hist_cap = 0;
-  std::vector<double*> ckpt;                // checkpoint states (3 x 9NN each)
-  double* d_seg = nullptr;                  // checkpoint mode segment history
-  size_t seg_cap = 0;
+  // checkpointed history: snapshot pool (3 x field x nshot each, kept across
+  // solves), the slice buffer of one segment, the schedule of the last forward
+  std::vector<double*> snap;
+  size_t snap_elems = 0;
+  double* d_seg = nullptr;
+  int h_seglen = 0, h_nseg = 0, h_nsnap = 0;  // segment length, segments, snapshot slots
+  bool h_store = false;                       // the plan keeps every slice (store mode)
+  RevolvePlan h_plan;
+  double h_budget = 0.0;                      // HBM bytes for the revolve history (0: automatic)
+  // adjoint state buffers used while the forward state is recomputed (6 fields)
+  double* adj_buf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // receivers grouped by equation for the misfit (receivers sharing one add first)
+  int mis_ngrp = 0;
+  int* d_mis_ptr = nullptr;
+  int* d_mis_idx = nullptr;
+  // staging of host vectors for hr_apply (neq each)
+  double *d_ax = nullptr, *d_ay = nullptr;
+  // capacities of the grow-only device allocations, by the address of their pointer field
+  std::unordered_map<void*, size_t> caps;
   // forward sources for recomputation
+  bool have_src = false;
   std::vector<int> src_tile_ptr_h;
   int* d_src_tile_ptr = nullptr;
   long long* d_src_node = nullptr;
@@ -169,13 +188,29 @@ int fail(hr_ctx* c, int code, const char* fmt, ...) {
     if (_e != hipSuccess) return fail(ctx, (int)_e, "%s failed: %s", #call, hipGetErrorString(_e)); \
   } while (0)
 
+// Grow-only device allocation: a pointer field keeps its buffer while the
+// request fits, so repeated solves allocate nothing (contents are not kept on
+// growth).
 template <class T>
 int dalloc(hr_ctx* ctx, T** p, size_t n) {
-  if (*p) { (void)hipFree(*p); *p = nullptr; }
   if (n == 0) n = 1;
+  auto it = ctx->caps.find((void*)p);
+  if (*p && it != ctx->caps.end() && it->second >= n * sizeof(T)) return 0;
+  if (*p) { (void)hipFree(*p); *p = nullptr; }
   hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess) return fail(ctx, (int)e, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  if (e != hipSuccess) {
+    ctx->caps.erase((void*)p);
+    return fail(ctx, (int)e, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  }
+  ctx->caps[(void*)p] = n * sizeof(T);
   return 0;
+}
+
+template <class T>
+void dfree(hr_ctx* ctx, T** p) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  ctx->caps.erase((void*)p);
 }
 
 // host <-> device copies honouring HR_DEVICE_PTRS
@@ -453,7 +488,7 @@ int setup_tiles(hr_ctx* ctx) {
     if (int rc = upload(ctx, &ctx->d_tl[c], lists[c])) return rc;
   }
   // per-tile source lists of an earlier forward no longer match the tiles
-  if (ctx->d_src_tile_ptr) { (void)hipFree(ctx->d_src_tile_ptr); ctx->d_src_tile_ptr = nullptr; }
+  ctx->have_src = false;
   ctx->f_nstep = 0;
   ctx->f_history = HR_HISTORY_NONE;
   return 0;
@@ -535,7 +570,7 @@ int store_slice(hr_ctx* ctx, double* base, size_t rows, size_t row) {
 void forward_sources(hr_ctx* ctx, int n, Src3Spec sp4[4]) {
   for (int s = 0; s < 4; ++s) {
     sp4[s] = Src3Spec();
-    if (!ctx->d_src_tile_ptr) continue;
+    if (!ctx->have_src) continue;
     sp4[s].lists = &ctx->src3_fwd;
     sp4[s].mode = 1;
     sp4[s].step = n;
@@ -582,8 +617,45 @@ double hr_time_function(double T, int lcase, double fr) {
       return 0.0;
     }
     case 3: return std::exp(-(T - 0.06) * (T - 0.06) / 0.00040);
+    case 4:
+    case 5: {
+      // Chopra's cosine wave and its derivative: t_d = 10 s, w_f = 3 rad/s, an
+      // integer number of cycles nc (truncated), period stretched to fit t_d
+      const double t_d = 10.0;
+      const int nc = (int)(t_d / (2.0 * pi / 3.00) - 0.50);
+      const double T_f = t_d / ((double)nc + 0.50);
+      const double tn = (double)nc * T_f;
+      if (T < T_f / 2.0)
+        return lcase == 4 ? 0.5 * (1.0 - std::cos(2.0 * pi * T / T_f)) : (pi / T_f) * std::sin(2.0 * pi * T / T_f);
+      if (T < tn)
+        return lcase == 4 ? std::cos(2.0 * pi * (T - T_f / 2.0) / T_f)
+                          : (-2.0 * pi / T_f) * std::sin(2.0 * pi * (T - T_f / 2.0) / T_f);
+      if (T <= t_d)
+        return lcase == 4 ? 0.50 * (1.0 - std::cos(2.0 * pi * (T - tn) / T_f)) - 1.0
+                          : (pi / T_f) * std::sin(2.0 * pi * (T - tn) / T_f);
+      return 0.0;
+    }
+    case 6: {
+      // Greenberg's delta: Cauchy density of scale 1/10 centred at 5e-4 s
+      const double a = 10.0 * (T - 0.0005);
+      return 10.0 / (pi * (1.0 + a * a));
+    }
+    case 7: {
+      // time derivative of the Ricker pulse of case 2
+      const double wr = 2.0 * pi * fr;
+      const double t_max = 6.0 * std::sqrt(6.0) / wr;
+      const double u = wr * T - 3.0 * std::sqrt(6.0);
+      if (T < t_max) {
+        const double e = std::exp(-0.25 * (u * u));
+        const double f1 = 0.25 * 2.0 * wr * u * e + (0.25 * (u * u) - 0.5) * (-0.25 * 2.0 * wr * u) * e;
+        return f1 / (0.50 + 13.0 * std::exp(-13.5));
+      }
+      return 0.0;
+    }
     case 8: return 1.0;
-    default: return 0.0;
+    case 9: return 0.0;
+    case 10: return T <= 0.0005 ? T / 0.0005 : 1.0;   // smooth Heaviside
+    default: return std::nan("");                      // the reference stops here
   }
 }
 
@@ -662,6 +734,11 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
   }
   if (int rc = dalloc(ctx, &ctx->tmp, nf)) return rc;
   if (int rc = dalloc(ctx, &ctx->tmp2, nf)) return rc;
+  if (!(ctx->flags & HR_DEVICE_PTRS)) {
+    // staging of host vectors for hr_apply, so that the product allocates nothing per call
+    if (int rc = dalloc(ctx, &ctx->d_ax, (size_t)ctx->neq)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_ay, (size_t)ctx->neq)) return rc;
+  }
   HIPC(hipMemset(ctx->tmp, 0, nf * sizeof(double)));
   HIPC(hipMemset(ctx->tmp2, 0, nf * sizeof(double)));
   if (int rc = zero_state(ctx)) return rc;
@@ -676,7 +753,9 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   F(ctx->d_node); F(ctx->d_tab); F(ctx->d_ftab);
   for (int a = 0; a < 3; ++a) { F(ctx->x[a]); F(ctx->y[a]); F(ctx->gs[a]); F(ctx->shell[a].l); F(ctx->shell[a].r); }
   F(ctx->tmp); F(ctx->tmp2); F(ctx->d_traces); F(ctx->d_rec_slot); F(ctx->d_hist); F(ctx->d_seg);
-  for (double* p : ctx->ckpt) F(p);
+  for (double* p : ctx->snap) F(p);
+  for (int a = 0; a < 6; ++a) F(ctx->adj_buf[a]);
+  F(ctx->d_mis_ptr); F(ctx->d_mis_idx); F(ctx->d_ax); F(ctx->d_ay);
   F(ctx->d_src_tile_ptr); F(ctx->d_src_node); F(ctx->d_src_comp); F(ctx->d_src_m); F(ctx->d_src_val);
   F(ctx->d_rec_tile_ptr); F(ctx->d_rec_node); F(ctx->d_rec_comp); F(ctx->d_rec_m); F(ctx->d_rec_perm);
   F(ctx->d_obs); F(ctx->d_gl); F(ctx->d_gm); F(ctx->d_part); F(ctx->d_scalar); F(ctx->d_tr_sorted);
@@ -809,6 +888,9 @@ int hr_set_material(hr_ctx* ctx, const double* lam, const double* mu, const doub
     if (int rc = upload(ctx, &ctx->d_tab, tb)) return rc;
   }
   ctx->have_material = true;
+  // the history and traces of an earlier forward belong to the old operator
+  ctx->f_nstep = 0;
+  ctx->f_history = HR_HISTORY_NONE;
   return 0;
 }
 
@@ -839,8 +921,8 @@ int hr_reset_scaling(hr_ctx* ctx, int which) {
   if (!ctx || which < 0 || which > 2) return fail(ctx, 1, "bad operator index");
   Shell& S = ctx->shell[which];
   S.s = 1.0;
-  if (S.l) { (void)hipFree(S.l); S.l = nullptr; }
-  if (S.r) { (void)hipFree(S.r); S.r = nullptr; }
+  dfree(ctx, &S.l);
+  dfree(ctx, &S.r);
   return 0;
 }
 
@@ -870,14 +952,15 @@ int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const doubl
   const Shell& S = ctx->shell[which];
   const long long neq = ctx->neq;
   const bool dev = ctx->flags & HR_DEVICE_PTRS;
-  double *dx = nullptr, *dy = nullptr, *dz = nullptr;
-  if (dev) { dx = (double*)x; dy = (double*)y; dz = z; }
-  else {
-    if (int rc = dalloc(ctx, &dx, neq)) return rc;
-    if (int rc = dalloc(ctx, &dy, neq)) return rc;
-    dz = dy;
-    HIPC(hipMemcpyAsync(dx, x, neq * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIPC(hipMemcpyAsync(dy, y, neq * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  const double *dx = x, *dy = y;
+  double* dz = z;
+  if (!dev) {
+    // host vectors through the staging buffers allocated with the context
+    HIPC(hipMemcpyAsync(ctx->d_ax, x, neq * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->d_ay, y, neq * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    dx = ctx->d_ax;
+    dy = ctx->d_ay;
+    dz = ctx->d_ay;
   }
   // A_eff x = s l (A (r x)) ; A_eff^T x = s r (A^T (l x))
   const double* pre = transpose ? S.l : S.r;
@@ -894,8 +977,6 @@ int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const doubl
   if (!dev) {
     HIPC(hipMemcpyAsync(z, dz, neq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
-    (void)hipFree(dx);
-    (void)hipFree(dy);
   }
   return 0;
 }
@@ -933,14 +1014,14 @@ int cd_step(hr_ctx* ctx, double dt, int n) {
   A.in[0] = ctx->x[1]; A.in[1] = ctx->x[0]; A.in[2] = ctx->x[2];
   A.xnew[0] = ctx->y[1]; A.xnew[1] = ctx->y[2];
   Src3Spec sp;
-  if (ctx->d_src_tile_ptr) {
+  if (ctx->have_src) {
     A.nsrc_tiles = 1;
     sp.lists = &ctx->src3_fwd;
     sp.mode = 1;
     sp.step = n;
     sp.fcol = 0;
   }
-  if (int rc = launch_stage(ctx, A, ctx->d_src_tile_ptr ? &sp : nullptr)) return rc;
+  if (int rc = launch_stage(ctx, A, ctx->have_src ? &sp : nullptr)) return rc;
   double* nm = ctx->x[0];
   double* his = ctx->x[2];
   ctx->x[0] = ctx->x[1];
@@ -948,6 +1029,107 @@ int cd_step(hr_ctx* ctx, double dt, int n) {
   ctx->x[2] = ctx->y[2];
   ctx->y[1] = nm;
   ctx->y[2] = his;
+  return 0;
+}
+
+// Bytes of one history slice (regular-domain displacement of every shot) and
+// of one snapshot of the RK4 state (x1, x2, x3 of every shot).
+size_t slice_bytes(const hr_ctx* ctx, int nshot) { return 3 * (size_t)ctx->nbox * nshot * sizeof(double); }
+size_t state_bytes(const hr_ctx* ctx, int nshot) { return 3 * field_size(ctx) * nshot * sizeof(double); }
+
+// Plan the forward history for the adjoint: every slice (store), uniform
+// segment checkpoints, or the binomial schedule that fits the HBM budget with
+// the fewest recomputed steps.  Sets h_store, h_seglen, h_nseg, h_nsnap, h_plan.
+int plan_history(hr_ctx* ctx, int history, int segment, int nstep, int nshot) {
+  ctx->h_store = false;
+  ctx->h_seglen = ctx->h_nseg = ctx->h_nsnap = 0;
+  if (history == HR_HISTORY_NONE) return 0;
+  if (history == HR_HISTORY_STORE) {
+    ctx->h_store = true;
+    return 0;
+  }
+  const double sl = (double)slice_bytes(ctx, nshot), st = (double)state_bytes(ctx, nshot);
+  if (history == HR_HISTORY_CHECKPOINT) {
+    const int L = std::min(segment, nstep);
+    const int m = (nstep + L - 1) / L;
+    ctx->h_seglen = L;
+    ctx->h_nseg = m;
+    ctx->h_plan.build(m, m - 1);          // a snapshot at every segment start
+    ctx->h_nsnap = (int)ctx->h_plan.chain().size();
+    return 0;
+  }
+  // HR_HISTORY_REVOLVE
+  double budget = ctx->h_budget;
+  if (!(budget > 0.0)) {
+    size_t fr = 0, tot = 0;
+    HIPC(hipMemGetInfo(&fr, &tot));
+    double held = 0.0;
+    for (auto& kv : ctx->caps)
+      if (kv.first == (void*)&ctx->d_hist || kv.first == (void*)&ctx->d_seg) held += (double)kv.second;
+    held += (double)ctx->snap.size() * (double)ctx->snap_elems * sizeof(double);
+    budget = 0.5 * ((double)fr + held);
+  }
+  if ((double)(nstep + 1) * sl <= budget) {
+    ctx->h_store = true;                  // the whole history fits: no recomputation
+    return 0;
+  }
+  long long best = -1;
+  int bestL = 0, bests = 0;
+  for (int L = 4; L <= 4096; L *= 2) {
+    if (L >= nstep) break;
+    const int m = (nstep + L - 1) / L;
+    if (m > 2048) continue;
+    const double rem = budget - (double)(L + 1) * sl;
+    if (rem < 0.0) continue;
+    const int s = (int)std::min<double>(std::floor(rem / st), (double)(m - 1));
+    RevolvePlan P;
+    P.build(m, s);
+    const long long steps = (long long)nstep + (long long)L * P.cost();
+    if (best < 0 || steps < best) { best = steps; bestL = L; bests = s; }
+  }
+  if (best < 0)
+    return fail(ctx, 1, "history budget of %.3g bytes is too small (one slice is %.3g bytes, one snapshot %.3g)",
+                budget, sl, st);
+  ctx->h_seglen = bestL;
+  ctx->h_nseg = (nstep + bestL - 1) / bestL;
+  ctx->h_plan.build(ctx->h_nseg, bests);
+  ctx->h_nsnap = (int)ctx->h_plan.chain().size();
+  return 0;
+}
+
+// snapshot pool and slice buffer of the planned history (allocated before the time loop)
+int ensure_history_buffers(hr_ctx* ctx, int nshot) {
+  if (ctx->h_store || ctx->h_seglen == 0) return 0;
+  const size_t elems = 3 * field_size(ctx) * nshot;
+  if (elems > ctx->snap_elems) {
+    for (double* p : ctx->snap) (void)hipFree(p);
+    ctx->snap.clear();
+    ctx->snap_elems = elems;
+  }
+  // the schedule never holds more snapshots than its slots
+  const size_t need = (size_t)std::max(ctx->h_plan.s, ctx->h_nsnap);
+  while (ctx->snap.size() < need) {
+    double* p = nullptr;
+    hipError_t e = hipMalloc((void**)&p, elems * sizeof(double));
+    if (e != hipSuccess) return fail(ctx, (int)e, "snapshot allocation failed: %s", hipGetErrorString(e));
+    ctx->snap.push_back(p);
+  }
+  const size_t seg = (size_t)nshot * (ctx->h_seglen + 1) * 3 * (size_t)ctx->nbox;
+  if (int rc = dalloc(ctx, &ctx->d_seg, seg)) return rc;
+  return 0;
+}
+
+// snapshot <-> the three state fields of every shot
+int snap_save(hr_ctx* ctx, double* const* fields, double* snap, int nshot) {
+  const size_t nfs = field_size(ctx) * nshot;
+  for (int a = 0; a < 3; ++a)
+    HIPC(hipMemcpyAsync(snap + a * nfs, fields[a], nfs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  return 0;
+}
+int snap_load(hr_ctx* ctx, double* const* fields, const double* snap, int nshot) {
+  const size_t nfs = field_size(ctx) * nshot;
+  for (int a = 0; a < 3; ++a)
+    HIPC(hipMemcpyAsync(fields[a], snap + a * nfs, nfs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   return 0;
 }
 
@@ -964,7 +1146,7 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
   if (nsrc < 0 || nrec < 0) return fail(ctx, 1, "negative counts");
   if (nsrc > 0 && (!src_eq || !src_val || (nshot > 1 && !src_shot))) return fail(ctx, 1, "null source arrays");
   if (nrec > 0 && !rec_eq) return fail(ctx, 1, "null receiver array");
-  if (history < 0 || history > 2) return fail(ctx, 1, "bad history mode");
+  if (history < 0 || history > 3) return fail(ctx, 1, "bad history mode");
   if (history == HR_HISTORY_CHECKPOINT && segment < 1) return fail(ctx, 1, "checkpoint segment must be >= 1");
   const bool dev = ctx->flags & HR_DEVICE_PTRS;
   // host copies of the small index arrays
@@ -1037,9 +1219,9 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
     h3.tile_ptr = ctx->d_src_tile_ptr; h3.node = ctx->d_src_node; h3.comp = ctx->d_src_comp;
     h3.m = ctx->d_src_m; h3.val = ctx->d_src_val; h3.ftab = ctx->d_ftab; h3.nstep = nstep; h3.ntiles = nt;
     ctx->src3_fwd = h3;
-  } else if (ctx->d_src_tile_ptr) {
-    (void)hipFree(ctx->d_src_tile_ptr);
-    ctx->d_src_tile_ptr = nullptr;
+    ctx->have_src = true;
+  } else {
+    ctx->have_src = false;
   }
   // receivers
   ctx->f_nshot = nshot; ctx->f_nstep = nstep; ctx->f_dt = dt; ctx->f_nrec = nrec; ctx->f_history = history;
@@ -1047,22 +1229,35 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
   ctx->rec_slot_h.resize(nrec);
   for (int q = 0; q < nrec; ++q) ctx->rec_slot_h[q] = ctx->eq_slot[req[q]];
   if (int rc = upload(ctx, &ctx->d_rec_slot, ctx->rec_slot_h)) return rc;
+  {
+    // receivers grouped by equation, groups in ascending equation order (misfit)
+    std::vector<int> ord(nrec), gptr(1, 0);
+    for (int q = 0; q < nrec; ++q) ord[q] = q;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return req[a] < req[b]; });
+    for (int q = 0; q < nrec; ++q)
+      if (q + 1 == nrec || req[ord[q + 1]] != req[ord[q]]) gptr.push_back(q + 1);
+    ctx->mis_ngrp = (int)gptr.size() - 1;
+    if (int rc = upload(ctx, &ctx->d_mis_ptr, gptr)) return rc;
+    if (int rc = upload(ctx, &ctx->d_mis_idx, ord)) return rc;
+  }
   const size_t rows = (size_t)nstep + 1;
   const size_t ntr = (size_t)nshot * rows * (nrec > 0 ? nrec : 1);
   if (ctx->traces_cap < ntr) {
     if (int rc = dalloc(ctx, &ctx->d_traces, ntr)) return rc;
     ctx->traces_cap = ntr;
   }
-  // history storage
+  // history storage, all of it allocated before the time loop
   const size_t slice = 3 * (size_t)ctx->nbox;
-  for (double* p : ctx->ckpt) (void)hipFree(p);
-  ctx->ckpt.clear();
-  if (history == HR_HISTORY_STORE) {
-    const size_t need = (size_t)nshot * rows * slice;
-    if (ctx->hist_cap < need) {
-      if (int rc = dalloc(ctx, &ctx->d_hist, need)) return rc;
-      ctx->hist_cap = need;
+  if (scheme == 0) {
+    if (int rc = plan_history(ctx, history, segment, nstep, nshot)) return rc;
+    if (ctx->h_store) {
+      const size_t need = (size_t)nshot * rows * slice;
+      if (ctx->hist_cap < need) {
+        if (int rc = dalloc(ctx, &ctx->d_hist, need)) return rc;
+        ctx->hist_cap = need;
+      }
     }
+    if (int rc = ensure_history_buffers(ctx, nshot)) return rc;
   }
   if (int rc = zero_state(ctx)) return rc;
   if (scheme == 1) {
@@ -1080,25 +1275,37 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
     return 0;
   }
   const double sgn[3] = {1.0, 1.0, 1.0};
-  const size_t nfs = field_size(ctx) * nshot;
+  // checkpointed history: the forward takes the first chain of snapshots and
+  // keeps the slices of the last segment
+  const int L = ctx->h_seglen;
+  std::vector<int> chain_slot;
+  if (L > 0) {
+    chain_slot.assign(ctx->h_nseg, -1);
+    const std::vector<int> ch = ctx->h_plan.chain();
+    for (size_t i = 0; i < ch.size(); ++i) chain_slot[ch[i]] = (int)i;
+  }
+  const int last0 = L > 0 ? (ctx->h_nseg - 1) * L : 0;
+  const size_t seg_rows = (size_t)L + 1;
   for (int n = 0; n < nstep; ++n) {
     // traces and history slice of the state at the start of step n, queued beside its stage 1
     ctx->deferred.push_back([ctx, n]() { return record_step(ctx, n); });
-    if (history == HR_HISTORY_STORE)
+    if (ctx->h_store)
       ctx->deferred.push_back([ctx, rows, n]() { return store_slice(ctx, ctx->d_hist, rows, (size_t)n); });
-    if (history == HR_HISTORY_CHECKPOINT && n % segment == 0) {
-      double* cp = nullptr;
-      if (int rc = dalloc(ctx, &cp, 3 * nfs)) return rc;
-      ctx->ckpt.push_back(cp);
-      for (int a = 0; a < 3; ++a)
-        HIPC(hipMemcpyAsync(cp + a * nfs, ctx->x[a], nfs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if (L > 0) {
+      if (n % L == 0 && chain_slot[n / L] >= 0)
+        if (int rc = snap_save(ctx, ctx->x, ctx->snap[chain_slot[n / L]], nshot)) return rc;
+      if (n >= last0)
+        ctx->deferred.push_back([ctx, seg_rows, n, last0]() {
+          return store_slice(ctx, ctx->d_seg, seg_rows, (size_t)(n - last0));
+        });
     }
     Src3Spec sp4[4];
     forward_sources(ctx, n, sp4);
     if (int rc = rk4_step(ctx, dt, +1, 0, sgn, sp4)) return rc;
   }
+  if (L > 0) if (int rc = store_slice(ctx, ctx->d_seg, seg_rows, (size_t)(nstep - last0))) return rc;
   if (int rc = record_step(ctx, nstep)) return rc;
-  if (history == HR_HISTORY_STORE) if (int rc = store_slice(ctx, ctx->d_hist, rows, (size_t)nstep)) return rc;
+  if (ctx->h_store) if (int rc = store_slice(ctx, ctx->d_hist, rows, (size_t)nstep)) return rc;
   if (traces && nrec) {
     HIPC(hipMemcpyAsync(traces, ctx->d_traces, (size_t)nshot * rows * nrec * sizeof(double),
                         dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
@@ -1149,7 +1356,8 @@ int misfit_all(hr_ctx* ctx, double dt, const double* dobs, double* out) {
   if (int rc = dalloc(ctx, &ctx->d_part, (size_t)nshot * rows + nshot)) return rc;
   double* per_shot = ctx->d_part + (size_t)nshot * rows;
   for (int q = 0; q < nshot; ++q)
-    HIPC(launch_misfit(nstep, nrec, ctx->d_traces + (size_t)q * rows * nrec, dobs + (size_t)q * rows * nrec, dt,
+    HIPC(launch_misfit(nstep, ctx->mis_ngrp, ctx->d_mis_ptr, ctx->d_mis_idx, nrec,
+                       ctx->d_traces + (size_t)q * rows * nrec, dobs + (size_t)q * rows * nrec, dt,
                        ctx->d_part + (size_t)q * rows, per_shot + q, ctx->stream));
   HIPC(launch_sum(nshot, per_shot, out, ctx->stream));
   return 0;
@@ -1254,61 +1462,37 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
   HIPC(hipMemsetAsync(ctx->d_gl, 0, ctx->nbox * sizeof(double), ctx->stream));
   HIPC(hipMemsetAsync(ctx->d_gm, 0, ctx->nbox * sizeof(double), ctx->stream));
   const size_t slice = 3 * (size_t)ctx->nbox;
-  const int seg = ctx->f_segment;
-  const size_t seg_rows = (size_t)seg + 1;
-  if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
-    const size_t need = (size_t)nshot * seg_rows * slice;
-    if (ctx->seg_cap < need) {
-      if (int rc = dalloc(ctx, &ctx->d_seg, need)) return rc;
-      ctx->seg_cap = need;
-    }
-  }
   const double sgn[3] = {-1.0, 1.0, -1.0};
   const size_t nf = field_size(ctx);
   const size_t nfs = nf * nshot;
-  long long recomputed = 0;
-  int seg_lo = -1;                       // history slices [seg_lo, seg_lo + seg] live in d_seg
-  // adjoint state starts from rest; the forward-state buffers are reused
+  const int L = ctx->h_seglen;
+  const size_t seg_rows = (size_t)L + 1;
+  // adjoint state starts from rest
   if (int rc = zero_state(ctx)) return rc;
-  // keep the adjoint state apart from the recomputation state: use extra buffers
-  double* adj[3];
-  double* adjy[3];
-  std::vector<double*> extra;
-  if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
-    for (int a = 0; a < 6; ++a) {
-      double* p = nullptr;
-      if (int rc = dalloc(ctx, &p, nfs)) return rc;
-      HIPC(hipMemsetAsync(p, 0, nfs * sizeof(double), ctx->stream));
-      extra.push_back(p);
-    }
-    for (int a = 0; a < 3; ++a) { adj[a] = extra[a]; adjy[a] = extra[3 + a]; }
+  // the recomputed forward state (F, FY) lives in the state buffers; the adjoint
+  // state (P, PY) in buffers of its own while the forward is recomputed
+  double *F[3], *FY[3], *P[3], *PY[3];
+  for (int a = 0; a < 3; ++a) { F[a] = ctx->x[a]; FY[a] = ctx->y[a]; }
+  if (ctx->h_store) {
+    for (int a = 0; a < 3; ++a) { P[a] = F[a]; PY[a] = FY[a]; }
   } else {
-    for (int a = 0; a < 3; ++a) { adj[a] = ctx->x[a]; adjy[a] = ctx->y[a]; }
+    for (int a = 0; a < 6; ++a) {
+      if (int rc = dalloc(ctx, &ctx->adj_buf[a], nfs)) return rc;
+      HIPC(hipMemsetAsync(ctx->adj_buf[a], 0, nfs * sizeof(double), ctx->stream));
+    }
+    for (int a = 0; a < 3; ++a) { P[a] = ctx->adj_buf[a]; PY[a] = ctx->adj_buf[3 + a]; }
   }
-  int status = 0;
-  for (int n = nstep; n >= 1 && !status; --n) {
+  auto use = [ctx](double* const* x, double* const* y) {
+    for (int a = 0; a < 3; ++a) { ctx->x[a] = x[a]; ctx->y[a] = y[a]; }
+  };
+  auto keep = [ctx](double** x, double** y) {
+    for (int a = 0; a < 3; ++a) { x[a] = ctx->x[a]; y[a] = ctx->y[a]; }
+  };
+  // adjoint step n, with the gradient of step n from history slice uh_row of uh
+  auto adjoint_step = [&](int n, const double* uh, size_t uh_rows, size_t uh_row) -> int {
     if (n <= nstep - 1) {
-      const double* uh = nullptr;
-      size_t uh_rows = rows, uh_row = (size_t)n;
-      if (ctx->f_history == HR_HISTORY_STORE) uh = ctx->d_hist;
-      else {
-        if (seg_lo < 0 || n < seg_lo || n > seg_lo + seg) {
-          // recompute the segment containing n: restore its checkpoint, step forward
-          const int c = (n - 1) / seg;
-          const int s0 = c * seg, s1 = std::min(s0 + seg, nstep);
-          double* cp = ctx->ckpt[c];
-          for (int a = 0; a < 3; ++a)
-            HIPC(hipMemcpyAsync(ctx->x[a], cp + a * nfs, nfs * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-          if (int rc = run_forward_steps(ctx, s0, s1, ctx->d_seg, seg_rows)) { status = rc; break; }
-          recomputed += s1 - s0;
-          seg_lo = s0;
-        }
-        uh = ctx->d_seg;
-        uh_rows = seg_rows;
-        uh_row = (size_t)(n - seg_lo);
-      }
-      // queued beside stage 1 of the adjoint step, which reads but does not write adj
-      double* const a1 = adj[1];
+      // queued beside stage 1 of the adjoint step, which reads but does not write p
+      double* const a1 = P[1];
       ctx->deferred.push_back([ctx, nshot, uh, uh_rows, uh_row, slice, a1, nf, dt]() -> int {
         for (int q = 0; q < nshot; ++q)
           HIPC(launch_gradient(ctx->g, ctx->box0, ctx->boxn, uh + ((size_t)q * uh_rows + uh_row) * slice,
@@ -1318,30 +1502,62 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     }
     // misfit sources at t_n (stage 1), t_{n-1} (stage 4), mean (stages 2, 3)
     Src3Spec sp4[4];
-    for (int s = 0; s < 4; ++s) {
-      sp4[s] = Src3Spec();
+    for (int st = 0; st < 4; ++st) {
+      sp4[st] = Src3Spec();
       if (!nrec) continue;
-      sp4[s].lists = &ctx->src3_adj;
-      sp4[s].mode = 2;
-      if (s == 0) { sp4[s].ra = n; sp4[s].ca = 1.0; }
-      else if (s == 3) { sp4[s].ra = n - 1; sp4[s].ca = 1.0; }
-      else { sp4[s].ra = n; sp4[s].ca = 0.5; sp4[s].rb = n - 1; sp4[s].cb = 0.5; }
+      sp4[st].lists = &ctx->src3_adj;
+      sp4[st].mode = 2;
+      if (st == 0) { sp4[st].ra = n; sp4[st].ca = 1.0; }
+      else if (st == 3) { sp4[st].ra = n - 1; sp4[st].ca = 1.0; }
+      else { sp4[st].ra = n; sp4[st].ca = 0.5; sp4[st].rb = n - 1; sp4[st].cb = 0.5; }
     }
-    // adjoint step on adj -> adjy
-    double* fx[3] = {ctx->x[0], ctx->x[1], ctx->x[2]};
-    double* fy[3] = {ctx->y[0], ctx->y[1], ctx->y[2]};
-    for (int a = 0; a < 3; ++a) { ctx->x[a] = adj[a]; ctx->y[a] = adjy[a]; }
-    if (int rc = rk4_step(ctx, dt, -1, 1, sgn, sp4)) { status = rc; }
-    for (int a = 0; a < 3; ++a) { adj[a] = ctx->x[a]; adjy[a] = ctx->y[a]; }
-    if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
-      for (int a = 0; a < 3; ++a) { ctx->x[a] = fx[a]; ctx->y[a] = fy[a]; }
+    use(P, PY);
+    const int rc = rk4_step(ctx, dt, -1, 1, sgn, sp4);
+    keep(P, PY);
+    return rc;
+  };
+  long long recomputed = 0;
+  int status = 0;
+  if (ctx->h_store) {
+    for (int n = nstep; n >= 1 && !status; --n) status = adjoint_step(n, ctx->d_hist, rows, (size_t)n);
+  } else {
+    // the reverse sweep of the binomial schedule; the forward kept the slices of the last segment
+    const std::vector<RevAction> acts = ctx->h_plan.actions(true);
+    const int m = ctx->h_nseg;
+    auto forward = [&](int n0, int n1, double* hist) -> int {
+      use(F, FY);
+      const int rc = run_forward_steps(ctx, n0, n1, hist, seg_rows);
+      keep(F, FY);
+      recomputed += n1 - n0;
+      return rc;
+    };
+    for (const RevAction& act : acts) {
+      if (status) break;
+      switch (act.op) {
+        case REV_ZERO:
+          for (int a = 0; a < 3; ++a)
+            if (hipMemsetAsync(F[a], 0, nfs * sizeof(double), ctx->stream) != hipSuccess) status = fail(ctx, 3, "memset failed");
+          break;
+        case REV_RESTORE: status = snap_load(ctx, F, ctx->snap[act.a], nshot); break;
+        case REV_SNAP: status = snap_save(ctx, F, ctx->snap[act.a], nshot); break;
+        case REV_ADVANCE: status = forward(act.a * L, act.b * L, nullptr); break;
+        case REV_RECORD: status = forward(act.a * L, std::min((act.a + 1) * L, nstep), ctx->d_seg); break;
+        case REV_CONSUME: {
+          const int lo = std::max(act.a * L, 1), hi = act.a == m - 1 ? nstep : (act.a + 1) * L - 1;
+          for (int n = hi; n >= lo && !status; --n)
+            status = adjoint_step(n, ctx->d_seg, seg_rows, (size_t)(n - act.a * L));
+          break;
+        }
+        default: status = fail(ctx, 3, "bad schedule action");
+      }
     }
   }
-  if (ctx->f_history == HR_HISTORY_CHECKPOINT) {
-    HIPC(hipStreamSynchronize(ctx->stream));
-    for (double* p : extra) (void)hipFree(p);
+  // the state buffers go back to the forward triple (the adjoint ones are context-owned)
+  use(F, FY);
+  if (status) {
+    ctx->deferred.clear();
+    return status;
   }
-  if (status) return status;
   ctx->recompute = 1.0 + (double)recomputed / (double)nstep;
   // nodal outputs: zero, then -g on the box
   double *dgl = g_lambda, *dgm = g_mu;
@@ -1371,6 +1587,73 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
 }
 
 double hr_recompute_ratio(const hr_ctx* ctx) { return ctx ? ctx->recompute : 0.0; }
+
+int hr_set_history_budget(hr_ctx* ctx, double bytes) {
+  if (!ctx) return 1;
+  if (!(bytes >= 0.0)) return fail(ctx, 1, "history budget must be >= 0");
+  ctx->h_budget = bytes;
+  return 0;
+}
+
+int hr_get_history_plan(hr_ctx* ctx, int nstep, int nshot, int history, int segment, int* seglen, int* nsnap,
+                        double* bytes, double* ratio) {
+  if (!ctx) return 1;
+  if (nstep < 1 || nshot < 1 || history < 0 || history > 3) return fail(ctx, 1, "bad history arguments");
+  if (history == HR_HISTORY_CHECKPOINT && segment < 1) return fail(ctx, 1, "checkpoint segment must be >= 1");
+  // plan into a scratch copy of the fields the planner sets
+  const bool st0 = ctx->h_store;
+  const int l0 = ctx->h_seglen, m0 = ctx->h_nseg, s0 = ctx->h_nsnap;
+  RevolvePlan p0 = ctx->h_plan;
+  int rc = plan_history(ctx, history, segment, nstep, nshot);
+  if (rc == 0) {
+    const double sl = (double)slice_bytes(ctx, nshot), st = (double)state_bytes(ctx, nshot);
+    double b = 0.0, r = 1.0;
+    int L = 0, ns = 0;
+    if (ctx->h_store) {
+      b = (nstep + 1) * sl;
+    } else if (ctx->h_seglen > 0) {
+      L = ctx->h_seglen;
+      ns = std::max(ctx->h_plan.s, ctx->h_nsnap);
+      b = ns * st + (L + 1) * sl;
+      // recomputation: every segment but the last once, plus the schedule's advances
+      const double rec = (double)std::max(0, nstep - (ctx->h_nseg - 1) * L);
+      r = 1.0 + ((double)nstep - rec + (double)L * (double)ctx->h_plan.cost()) / (double)nstep;
+    }
+    if (seglen) *seglen = L;
+    if (nsnap) *nsnap = ns;
+    if (bytes) *bytes = b;
+    if (ratio) *ratio = r;
+  }
+  ctx->h_store = st0;
+  ctx->h_seglen = l0;
+  ctx->h_nseg = m0;
+  ctx->h_nsnap = s0;
+  ctx->h_plan = p0;
+  return rc;
+}
+
+int hr_revolve_schedule(int nseg, int nsnap, int max_actions, int* actions, int* nactions, long long* advances) {
+  if (nseg < 1 || nsnap < 0 || !nactions) return 1;
+  RevolvePlan P;
+  P.build(nseg, nsnap);
+  const std::vector<RevAction> a = P.actions(true);
+  const std::vector<int> ch = P.chain();
+  *nactions = (int)(a.size() + ch.size());
+  if (advances) *advances = P.cost();
+  if (actions) {
+    if (max_actions < *nactions) return 2;
+    int k = 0;
+    // the forward's snapshots first (REV_SNAP, slot, segment), then the reverse sweep
+    for (size_t i = 0; i < ch.size(); ++i, ++k) {
+      actions[3 * k] = REV_SNAP; actions[3 * k + 1] = (int)i; actions[3 * k + 2] = ch[i];
+    }
+    for (const RevAction& x : a) {
+      actions[3 * k] = x.op; actions[3 * k + 1] = x.a; actions[3 * k + 2] = x.b;
+      ++k;
+    }
+  }
+  return 0;
+}
 
 }  // extern "C"
 

@@ -55,8 +55,9 @@ hipError_t launch_gradient_finish(const Geo& g, const int* box0, const int* boxn
 hipError_t launch_permute_cols(long long nrows, int ncols, const long long* perm, const double* src, double* dst,
                                hipStream_t s);
 hipError_t launch_sum(int n, const double* in, double* out, hipStream_t s);
-hipError_t launch_misfit(int nstep, int nrec, const double* tr, const double* ob, double dt, double* part,
-                         double* out, hipStream_t s);
+// receivers grouped by equation: group g = receivers gidx[gptr[g] .. gptr[g+1])
+hipError_t launch_misfit(int nstep, int ngrp, const int* gptr, const int* gidx, int nrec, const double* tr,
+                         const double* ob, double dt, double* part, double* out, hipStream_t s);
 // gradient post-processing (hr_reg.hip)
 long long reg_box_elements(const Geo& g);
 hipError_t launch_gradient_mass(const Geo& g, double* img, double* restr, hipStream_t s);

@@ -152,15 +152,25 @@ __global__ void permute_cols_kernel(long long nrows, int ncols, const long long*
   }
 }
 
-// Misfit partial sums per step: part[n] = sum_r (trace[n][r] - obs[n][r])^2
-__global__ void misfit_rows_kernel(int nstep, int nrec, const double* __restrict__ tr, const double* __restrict__ ob,
+// Misfit partial sums per step (Compute_Misfit, Solver_PETSc_Inverse.F90:633-650):
+// the residuals of receivers that share an equation are added first (VecSetValues
+// with ADD_VALUES), then the squares are summed in equation order (VecDot).
+// Group g holds the receivers gidx[gptr[g] .. gptr[g+1]), groups in ascending
+// equation order.
+__global__ void misfit_rows_kernel(int nstep, int ngrp, const int* __restrict__ gptr, const int* __restrict__ gidx,
+                                   int nrec, const double* __restrict__ tr, const double* __restrict__ ob,
                                    double* __restrict__ part) {
   const int nstp = blockIdx.x * blockDim.x + threadIdx.x;
   if (nstp > nstep) return;
+  const long long row = (long long)nstp * nrec;
   double c = 0.0;
-  for (int r = 0; r < nrec; ++r) {
-    const double d = tr[(long long)nstp * nrec + r] - ob[(long long)nstp * nrec + r];
-    c += d * d;
+  for (int g = 0; g < ngrp; ++g) {
+    double b = 0.0;
+    for (int j = gptr[g]; j < gptr[g + 1]; ++j) {
+      const int r = gidx[j];
+      b += tr[row + r] - ob[row + r];
+    }
+    c += b * b;
   }
   part[nstp] = c;
 }
@@ -251,9 +261,10 @@ hipError_t launch_sum(int n, const double* in, double* out, hipStream_t s) {
   hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(64), 0, s, n, in, out);
   return hipGetLastError();
 }
-hipError_t launch_misfit(int nstep, int nrec, const double* tr, const double* ob, double dt, double* part, double* out,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(misfit_rows_kernel, dim3((nstep + 1 + 127) / 128), dim3(128), 0, s, nstep, nrec, tr, ob, part);
+hipError_t launch_misfit(int nstep, int ngrp, const int* gptr, const int* gidx, int nrec, const double* tr,
+                         const double* ob, double dt, double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(misfit_rows_kernel, dim3((nstep + 1 + 127) / 128), dim3(128), 0, s, nstep, ngrp, gptr, gidx, nrec,
+                     tr, ob, part);
   hipLaunchKernelGGL(misfit_sum_kernel, dim3(1), dim3(64), 0, s, nstep, part, dt, out);
   return hipGetLastError();
 }

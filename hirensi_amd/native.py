@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libhirensi.so")
 
 HR_DEVICE_PTRS = 1
 HR_K, HR_C, HR_G = 0, 1, 2
-HR_HISTORY_NONE, HR_HISTORY_STORE, HR_HISTORY_CHECKPOINT = 0, 1, 2
+HR_HISTORY_NONE, HR_HISTORY_STORE, HR_HISTORY_CHECKPOINT, HR_HISTORY_REVOLVE = 0, 1, 2, 3
 HR_REG_TV, HR_REG_TN = 0, 1
 
 EXPORTS = (
@@ -22,7 +22,7 @@ EXPORTS = (
     "hr_reset_scaling", "hr_apply", "hr_time_function", "hr_forward", "hr_adjoint_gradient",
     "hr_misfit", "hr_recompute_ratio", "hr_set_tiling", "hr_set_profiling", "hr_get_kernel_stats",
     "hr_forward_shots", "hr_gradient_mass", "hr_regularization", "hr_regularize_gradient",
-    "hr_forward_cd",
+    "hr_forward_cd", "hr_set_history_budget", "hr_get_history_plan", "hr_revolve_schedule",
 )
 
 
@@ -96,6 +96,10 @@ def load():
     L.hr_gradient_mass.argtypes = [_vp, _dp, _dp]
     L.hr_regularization.argtypes = [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     L.hr_regularize_gradient.argtypes = [_vp, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.hr_set_history_budget.argtypes = [_vp, C.c_double]
+    L.hr_get_history_plan.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.hr_revolve_schedule.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     _lib = L

@@ -33,14 +33,15 @@ import numpy as np
 
 from . import native
 from .native import (HrError, HR_K, HR_C, HR_G, HR_HISTORY_NONE, HR_HISTORY_STORE, HR_HISTORY_CHECKPOINT,
-                     HR_REG_TV, HR_REG_TN)
+                     HR_HISTORY_REVOLVE, HR_REG_TV, HR_REG_TN)
 
 __all__ = ["Context", "ShellMat", "time_function", "time_table", "explicit_rk4_petsc_pml_3d",
            "explicit_rk4_petsc_pml_3d_shots",
            "explicit_rk4_petsc_pml_3d_adjoint_otd", "compute_misfit", "explicit_central_difference_3",
            "gradient_mass", "regularization",
            "regularize_gradient", "HrError", "HR_REG_TV", "HR_REG_TN",
-           "HR_K", "HR_C", "HR_G", "HR_HISTORY_NONE", "HR_HISTORY_STORE", "HR_HISTORY_CHECKPOINT"]
+           "HR_K", "HR_C", "HR_G", "HR_HISTORY_NONE", "HR_HISTORY_STORE", "HR_HISTORY_CHECKPOINT",
+           "HR_HISTORY_REVOLVE", "revolve_schedule"]
 
 
 def _ptr(a):
@@ -150,6 +151,19 @@ class Context:
     def recompute_ratio(self):
         return self.L.hr_recompute_ratio(self.h)
 
+    def set_history_budget(self, nbytes):
+        """HBM bytes for the HR_HISTORY_REVOLVE history (0: half the free memory)."""
+        self._check(self.L.hr_set_history_budget(self.h, float(nbytes)))
+
+    def history_plan(self, nstep, history=HR_HISTORY_REVOLVE, nshot=1, segment=64):
+        """(segment length, snapshot slots, HBM bytes, predicted recompute ratio)
+        of the history hr_forward would keep; segment length 0 = every slice."""
+        L, ns = C.c_int(), C.c_int()
+        b, r = C.c_double(), C.c_double()
+        self._check(self.L.hr_get_history_plan(self.h, int(nstep), int(nshot), int(history), int(segment),
+                                               C.byref(L), C.byref(ns), C.byref(b), C.byref(r)))
+        return L.value, ns.value, b.value, r.value
+
 
 class ShellMat:
     """PETSc ``Mat`` of one operator (K, C or G) as a matrix-free shell."""
@@ -194,8 +208,30 @@ class ShellMat:
 
 
 def time_function(t, lcase=2, fr=15.0):
-    """TIME_FUNCTION (IO.F90:541-733); lcase 2 = Ricker of central frequency fr."""
-    return native.load().hr_time_function(float(t), int(lcase), float(fr))
+    """TIME_FUNCTION (IO.F90:541-733): 1 sine, 2 Ricker of central frequency fr,
+    3 Gaussian, 4 Chopra's cosine wave, 5 its derivative, 6 Greenberg's delta,
+    7 derivative of the Ricker pulse, 8 one, 9 zero, 10 smooth Heaviside.
+    Any other case stops the reference (``ERROR IN TIME_FUNCTION``) and raises
+    here."""
+    v = native.load().hr_time_function(float(t), int(lcase), float(fr))
+    if v != v:
+        raise HrError(f"ERROR IN TIME_FUNCTION: no case {lcase}")
+    return v
+
+
+def revolve_schedule(nseg, nsnap):
+    """The binomial reverse schedule of ``nseg`` segments with ``nsnap``
+    snapshot slots (``hr_revolve_schedule``): (actions, advances), actions as
+    (op, a, b) rows, the forward's snapshots first."""
+    L = native.load()
+    n = C.c_int()
+    adv = C.c_longlong()
+    if L.hr_revolve_schedule(int(nseg), int(nsnap), 0, None, C.byref(n), C.byref(adv)) != 0:
+        raise HrError("bad revolve arguments")
+    out = np.zeros(3 * max(n.value, 1), dtype=np.int32)
+    if L.hr_revolve_schedule(int(nseg), int(nsnap), n.value, out.ctypes.data, C.byref(n), C.byref(adv)) != 0:
+        raise HrError("revolve schedule failed")
+    return out[:3 * n.value].reshape(-1, 3), adv.value
 
 
 def time_table(nstep, dt, lcase=2, fr=15.0):

@@ -47,6 +47,7 @@ typedef struct hr_ctx hr_ctx;
 #define HR_HISTORY_NONE 0
 #define HR_HISTORY_STORE 1        /* keep u on the regular-domain box for every step */
 #define HR_HISTORY_CHECKPOINT 2   /* keep the state every `segment` steps, recompute */
+#define HR_HISTORY_REVOLVE 3      /* binomial checkpoints within an HBM budget (hr_set_history_budget) */
 
 /* Structured box mesh: the ANSYS box + PML of the reference pre-processing
  * (step_1_Ansys_Mesh_Generator, step_2_20_To_27/Main.F90:447-521). */
@@ -176,6 +177,26 @@ int hr_regularize_gradient(hr_ctx* ctx, int iter, int method, double factor, con
 
 /* Forward steps recomputed per adjoint (1.0 with a stored history). */
 double hr_recompute_ratio(const hr_ctx* ctx);
+
+/* The forward history for the adjoint.  The reference keeps every regular-domain
+ * displacement slice in host memory (U_Store_Mapping, PETSc_Global_Objects_Inversion.F90:71);
+ * HR_HISTORY_REVOLVE keeps it in HBM within `bytes`: every slice if they fit,
+ * else snapshots of the RK4 state at segment starts placed by the binomial
+ * schedule of Griewank & Walther (revolve) plus one segment of slices, chosen
+ * for the fewest recomputed steps.  0 (the default) = half of the free device
+ * memory.  No counterpart in the reference, which has no recomputation. */
+int hr_set_history_budget(hr_ctx* ctx, double bytes);
+/* The plan hr_forward would use: segment length (0: every slice kept),
+ * snapshot slots, HBM bytes and the predicted recompute ratio. */
+int hr_get_history_plan(hr_ctx* ctx, int nstep, int nshot, int history, int segment, int* seglen, int* nsnap,
+                        double* bytes, double* ratio);
+/* The binomial reverse schedule of nseg segments with nsnap snapshot slots
+ * (host only, no context): actions as (op, a, b) triples, the forward's
+ * snapshots first (op 3 = snapshot slot a <- segment b), then the reverse sweep
+ * (0 zero state, 1 restore slot a, 2 advance segment a -> b, 3 snapshot slot a
+ * <- segment b, 4 record segment a, 5 consume segment a); advances = segment
+ * advances of the sweep.  Returns 2 when max_actions is too small. */
+int hr_revolve_schedule(int nseg, int nsnap, int max_actions, int* actions, int* nactions, long long* advances);
 
 /* Stage-kernel configuration: tile of 4 x ty elements in x-y (ty = 3), tz
  * elements per z-chunk, kernel version (3, the only one; 0 keeps it).  The

@@ -638,10 +638,56 @@ double time_function(double T, int lcase, double fr) {
       const double G_MEAN = 0.06, G_SDEV = 0.00040;   // L_Gaussian = 40
       return std::exp(-(T - G_MEAN) * (T - G_MEAN) / G_SDEV);
     }
+    case 4:
+    case 5: {
+      // Chopra's cosine wave (IO.F90:630-664): t_d = 10, w_f = 3; nc is an
+      // implicitly typed INTEGER, so the cycle count is truncated toward zero
+      const double t_d = 10.0, w_f = 3.00;
+      double T_f = 2.0 * pi / w_f;
+      const int nc = (int)(t_d / T_f - 0.50);
+      T_f = t_d / ((double)nc + 0.50);
+      if (lcase == 4) {
+        if (T < T_f / 2.0) return 0.5 * (1.0 - std::cos(2.0 * pi * T / T_f));
+        if (T >= T_f / 2.0 && T < (double)nc * T_f) return std::cos(2.0 * pi * (T - T_f / 2.0) / T_f);
+        if (T >= (double)nc * T_f && T <= t_d) return 0.50 * (1.0 - std::cos(2.0 * pi * (T - (double)nc * T_f) / T_f)) - 1.0;
+        return 0.0;
+      }
+      if (T < T_f / 2.0) return (pi / T_f) * std::sin(2.0 * pi * T / T_f);
+      if (T >= T_f / 2.0 && T < (double)nc * T_f) return (-2.0 * pi / T_f) * std::sin(2.0 * pi * (T - T_f / 2.0) / T_f);
+      if (T >= (double)nc * T_f && T <= t_d) return (pi / T_f) * std::sin(2.0 * pi * (T - (double)nc * T_f) / T_f);
+      return 0.0;
+    }
+    case 6: {
+      // Greenberg's delta function (IO.F90:668-672): k = 10 (INTEGER), t0 = 5e-4
+      const int k = 10;
+      const double t0 = 0.0005;
+      const double a = (double)k * (T - t0);
+      return (double)k / (pi * (1.0 + a * a));
+    }
+    case 7: {
+      // derivative of the Ricker pulse (IO.F90:676-690)
+      const double wr = 2.0 * pi * fr;
+      const double t_max = 6.0 * std::sqrt(6.0) / wr;
+      const double u = wr * T - 3.0 * std::sqrt(6.0);
+      const double up = wr;
+      if (T < t_max) {
+        const double FAC1 = 0.25 * 2.0 * up * u * std::exp(-0.25 * (u * u)) +
+                            (0.25 * (u * u) - 0.5) * (-0.25 * 2.0 * up * u) * std::exp(-0.25 * (u * u));
+        const double FAC2 = 0.50 + 13.0 * std::exp(-13.5);
+        return FAC1 / FAC2;
+      }
+      return 0.0;
+    }
     case 8: return 1.0;
     case 9: return 0.0;
+    case 10: {
+      // smooth Heaviside (IO.F90:708-715), t0 = 5e-4
+      const double t0 = 0.0005;
+      return T <= t0 ? T / t0 : 1.0;
+    }
   }
-  return 0.0;
+  // the reference stops on any other case (IO.F90:718-720)
+  return std::nan("");
 }
 
 // ========================================================== global system
@@ -1279,15 +1325,28 @@ void or_control(void* s, int nstep, double dt, i64 nstore, const i64* store_eq, 
   for (i64 k = 0; k < S->nnode; ++k) { g_lambda[k] = g_lambda[k] * -1.0; g_mu[k] = g_mu[k] * -1.0; }
 }
 
-// Compute_Misfit: J = sum_{n=NSTEP..1} 0.5 * |u_rec(t_n) - d_n|^2 * dt
-double or_misfit(int nstep, double dt, int nrec, const i64* rec_row, const double* u_store, i64 nstore,
-                 const double* dis_meas) {
+// Compute_Misfit (Solver_PETSc_Inverse.F90:633-650): per step the residuals
+// u_rec(t_n) - d_n are added into B_mis with ADD_VALUES at their equations, so
+// receivers that share an equation add before squaring, and VecDot sums the
+// squares in equation order; J = sum_{n=NSTEP..1} 0.5 * (B_mis . B_mis) * dt.
+// rec_key[q]: the equation of receiver q (ordering and grouping key).
+double or_misfit(int nstep, double dt, int nrec, const i64* rec_row, const i64* rec_key, const double* u_store,
+                 i64 nstore, const double* dis_meas) {
+  std::vector<int> ord(nrec);
+  for (int q = 0; q < nrec; ++q) ord[q] = q;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return rec_key[a] < rec_key[b]; });
   double cost = 0.0;
   for (int istep = nstep; istep >= 1; --istep) {
     double c = 0.0;
-    for (int q = 0; q < nrec; ++q) {
-      const double um = u_store[rec_row[q] + nstore * (i64)istep] - dis_meas[(i64)istep * nrec + q];
-      c += um * um;
+    for (int a = 0; a < nrec;) {
+      double b = 0.0;
+      int e = a;
+      for (; e < nrec && rec_key[ord[e]] == rec_key[ord[a]]; ++e) {
+        const int q = ord[e];
+        b += u_store[rec_row[q] + nstore * (i64)istep] - dis_meas[(i64)istep * nrec + q];
+      }
+      c += b * b;
+      a = e;
     }
     cost = cost + 0.5 * c * dt;
   }

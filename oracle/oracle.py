@@ -58,7 +58,7 @@ def lib():
         L.or_rk4_adjoint.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int64, _ip, _dp,
                                      C.c_int, _ip, _ip, _dp, _dp]
         L.or_control.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int64, _ip, _dp, _dp, _dp, _dp]
-        L.or_misfit.argtypes = [C.c_int, C.c_double, C.c_int, _ip, _dp, C.c_int64, _dp]
+        L.or_misfit.argtypes = [C.c_int, C.c_double, C.c_int, _ip, _ip, _dp, C.c_int64, _dp]
         L.or_misfit.restype = C.c_double
         L.or_element_r_mg.argtypes = [_dp, C.c_void_p, _dp, _dp]
         L.or_element_tv.argtypes = [_dp, _dp, _dp, C.c_void_p, _dp, _dp, _dp]
@@ -330,8 +330,12 @@ class System:
         return gl, gm
 
 
-def misfit(u_store, nstep, dt, rec_rows, dis_meas):
+def misfit(u_store, nstep, dt, rec_rows, dis_meas, rec_eq=None):
+    """Compute_Misfit.  rec_eq (default: the rows) orders and groups the
+    receivers: those sharing an equation add their residuals before squaring."""
     ns = u_store.shape[1]
-    return lib().or_misfit(int(nstep), float(dt), len(rec_rows), np.ascontiguousarray(rec_rows, dtype=np.int64),
+    rows = np.ascontiguousarray(rec_rows, dtype=np.int64)
+    key = rows if rec_eq is None else np.ascontiguousarray(rec_eq, dtype=np.int64)
+    return lib().or_misfit(int(nstep), float(dt), len(rows), rows, key,
                            np.ascontiguousarray(u_store.reshape(-1)), ns,
                            np.ascontiguousarray(dis_meas, dtype=np.float64).reshape(-1))

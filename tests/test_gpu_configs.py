@@ -1,0 +1,329 @@
+"""The HIP path on the benchmark configurations and the paths the small parity
+tests leave out (runs on the GPU):
+
+* non-cubic elements, h = (1.0, 1.25, 0.8), with uneven PML layers: operator
+  applies, forward traces and the lambda/mu gradient against the oracle, and
+  the device-pointer path (HR_DEVICE_PTRS, torch tensors) bit-identical to the
+  host-pointer path;
+* config 1 (16x16x8, PML 4, homogeneous) at its full 2000 steps and 15 Hz;
+* config 2 (64x64x32, PML 6, layered, 400 receivers) through the device-pointer
+  path that bench.py times, 250 steps against the oracle, and the full 4000
+  steps with stored, uniformly checkpointed and binomially checkpointed
+  (revolve, 4 GB budget) histories giving bit-identical gradients;
+* config 3 (48x48x24, 16 shots, 225 shared receivers) at reduced steps: the
+  batched launch against single shots, and one shot against the oracle;
+* two receivers on one equation: misfit and gradient follow ADD_VALUES.
+
+Bars: traces and gradients <= 1e-10 relative L2, misfit <= 1e-12 relative,
+applies <= 1e-13 of sum |A||x| + |y|, checkpointed vs stored bit-identical.
+"""
+import numpy as np
+import pytest
+
+from hirensi_amd import solver as S
+from hirensi_amd.mesh import BoxMesh, config1, config2, config3, receiver_dofs, surface_grid_nodes
+from oracle import oracle as O
+from tests.helpers import random_model, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+OPS = {S.HR_K: 0, S.HR_C: 1, S.HR_G: 2}
+
+
+def _torch():
+    import torch
+    return torch, torch.device("cuda:0")
+
+
+def _oracle_run(mesh, model, src_eq, src_val, rec_eq, nstep, dt, fr, dmeas=None, threads=0):
+    """Forward (+ adjoint, control, misfit when dmeas is given) on the oracle."""
+    sysm = O.System(mesh, model, nthreads=threads)
+    B = np.zeros(mesh.neq)
+    np.add.at(B, src_eq, src_val)
+    U = sysm.forward(B, nstep, dt, lcase=2, fr=fr)
+    pos = {int(e): r for r, e in enumerate(mesh.store_eq)}
+    rows = np.array([pos[int(e)] for e in rec_eq])
+    out = {"traces": U[:, rows].copy()}
+    if dmeas is not None:
+        P = sysm.adjoint(U, nstep, dt, rec_eq, dmeas)
+        out["gl"], out["gm"] = sysm.control(U, P, nstep, dt)
+        out["misfit"] = O.misfit(U, nstep, dt, rows, dmeas, rec_eq=rec_eq)
+    sysm.close()
+    return out
+
+
+# ------------------------------------------------------------ non-cubic elements
+ANISO = dict(shape=(8, 7, 6), npml=(1, 2, 2, 1, 2), h=(1.0, 1.25, 0.8))
+
+
+def _aniso_case():
+    mesh = BoxMesh(*ANISO["shape"], npml=ANISO["npml"], h=ANISO["h"])
+    model = random_model(mesh, seed=31)
+    rec_eq = receiver_dofs(mesh, surface_grid_nodes(mesh, 3, 3, -1.0, 1.0, -1.25, 1.25))
+    src_eq = np.array([mesh.dof(mesh.surface_node(0.0, 0.0), 2), mesh.dof(mesh.surface_node(0.5, 0.0), 0)],
+                      dtype=np.int64)
+    src_val = np.array([1.0, 0.4])
+    return mesh, model, rec_eq, src_eq, src_val
+
+
+@pytest.mark.parametrize("transpose", [0, 1])
+def test_anisotropic_apply_matches_oracle(transpose):
+    mesh, model, *_ = _aniso_case()
+    print(f"anisotropic apply case {ANISO} transpose={transpose}", flush=True)
+    ctx = S.Context(mesh, model)
+    sysm = O.System(mesh, model)
+    rng = np.random.default_rng(17)
+    for which, w in OPS.items():
+        A = sysm.csr(w)
+        At = A.T.tocsr() if transpose else A
+        x, y = rng.standard_normal(mesh.neq), rng.standard_normal(mesh.neq)
+        z = S.ShellMat(ctx, which)._apply(transpose, x, y, None)
+        bound = np.abs(At).dot(np.abs(x)) + np.abs(y)
+        assert np.max(np.abs(z - (y + At.dot(x))) / bound) < 1e-13, which
+
+
+def test_anisotropic_forward_adjoint_and_device_pointers():
+    torch, dev = _torch()
+    mesh, model, rec_eq, src_eq, src_val = _aniso_case()
+    nstep, dt, fr = 220, 1.0e-4, 60.0
+    print(f"anisotropic forward/adjoint case {ANISO} nstep={nstep}", flush=True)
+    true = random_model(mesh, seed=32)
+    dmeas = _oracle_run(mesh, true, src_eq, src_val, rec_eq, nstep, dt, fr)["traces"]
+    ref = _oracle_run(mesh, model, src_eq, src_val, rec_eq, nstep, dt, fr, dmeas=dmeas)
+    ftab = S.time_table(nstep, dt, 2, fr)
+    # host pointers
+    ctx = S.Context(mesh, model)
+    tr = S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, ftab, src_eq, src_val, rec_eq).copy()
+    gl, gm, mis = S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dmeas)
+    assert np.abs(ref["gm"]).max() > 0
+    assert rel_l2(tr, ref["traces"]) < 1e-10
+    assert rel_l2(gl, ref["gl"]) < 1e-10 and rel_l2(gm, ref["gm"]) < 1e-10
+    assert abs(mis - ref["misfit"]) <= 1e-12 * abs(ref["misfit"])
+    ctx.close()
+    # device pointers: the same kernels on the same inputs, bit for bit
+    T = lambda a, dt_=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt_, device=dev)
+    cd = S.Context(mesh, device_ptrs=True)
+    cd.set_material_ptrs(T(model.lam), T(model.mu), T(model.rho))
+    d_tr = torch.zeros((nstep + 1, len(rec_eq)), dtype=torch.float64, device=dev)
+    S.explicit_rk4_petsc_pml_3d(cd, nstep, dt, T(ftab), T(src_eq, torch.int64), T(src_val),
+                                T(rec_eq, torch.int64), traces=d_tr)
+    d_gl = torch.zeros(mesh.nnode, dtype=torch.float64, device=dev)
+    d_gm = torch.zeros_like(d_gl)
+    d_mis = torch.zeros(1, dtype=torch.float64, device=dev)
+    S.explicit_rk4_petsc_pml_3d_adjoint_otd(cd, nstep, dt, T(rec_eq, torch.int64), T(dmeas), d_gl, d_gm,
+                                            misfit_out=d_mis)
+    torch.cuda.synchronize()
+    assert np.array_equal(d_tr.cpu().numpy(), tr)
+    assert np.array_equal(d_gl.cpu().numpy(), gl) and np.array_equal(d_gm.cpu().numpy(), gm)
+    assert float(d_mis.item()) == mis
+    cd.close()
+
+
+# ----------------------------------------------------------------------- config 1
+@pytest.mark.timeout(600)
+def test_config1_full_length_15hz():
+    c = config1()
+    mesh, model = c["mesh"], c["model"]
+    rec_eq = receiver_dofs(mesh, c["rec_nodes"])
+    src_eq = np.array([mesh.dof(c["src_node"], 2)], dtype=np.int64)
+    nstep, dt, fr = c["nstep"], c["dt"], c["fr"]
+    print(f"config1 full forward neq={mesh.neq} nstep={nstep} fr={fr}", flush=True)
+    ctx = S.Context(mesh, model)
+    tr = S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, S.time_table(nstep, dt, 2, fr), src_eq, np.ones(1), rec_eq,
+                                     history=S.HR_HISTORY_NONE)
+    ref = _oracle_run(mesh, model, src_eq, np.ones(1), rec_eq, nstep, dt, fr)["traces"]
+    assert np.abs(ref).max() > 0
+    assert rel_l2(tr, ref) < 1e-10
+
+
+# ----------------------------------------------------------------------- config 2
+def _config2_setup():
+    true_cfg, start_cfg = config2(with_anomaly=True), config2(with_anomaly=False)
+    mesh = true_cfg["mesh"]
+    rec_eq = receiver_dofs(mesh, true_cfg["rec_nodes"])
+    src_eq = np.array([mesh.dof(true_cfg["src_node"], 2)], dtype=np.int64)
+    return mesh, true_cfg, start_cfg, rec_eq, src_eq
+
+
+class _DeviceSolve:
+    """The bench.py call sequence on device pointers."""
+
+    def __init__(self, mesh, rec_eq, src_eq, nstep, dt, fr):
+        torch, dev = _torch()
+        self.torch, self.dev = torch, dev
+        self.T = lambda a, dt_=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt_, device=dev)
+        self.mesh, self.nstep, self.dt = mesh, nstep, dt
+        self.ctx = S.Context(mesh, device_ptrs=True)
+        self.ftab = self.T(S.time_table(nstep, dt, 2, fr))
+        self.src = self.T(src_eq, torch.int64)
+        self.val = self.T(np.ones(len(src_eq)))
+        self.rec = self.T(rec_eq, torch.int64)
+        self.nrec = len(rec_eq)
+
+    def material(self, model):
+        self.ctx.set_material_ptrs(self.T(model.lam), self.T(model.mu), self.T(model.rho))
+
+    def forward(self, history=S.HR_HISTORY_NONE, segment=64):
+        tr = self.torch.zeros((self.nstep + 1, self.nrec), dtype=self.torch.float64, device=self.dev)
+        S.explicit_rk4_petsc_pml_3d(self.ctx, self.nstep, self.dt, self.ftab, self.src, self.val, self.rec,
+                                    history=history, segment=segment, traces=tr)
+        return tr
+
+    def gradient(self, dobs):
+        gl = self.torch.zeros(self.mesh.nnode, dtype=self.torch.float64, device=self.dev)
+        gm = self.torch.zeros_like(gl)
+        mis = self.torch.zeros(1, dtype=self.torch.float64, device=self.dev)
+        S.explicit_rk4_petsc_pml_3d_adjoint_otd(self.ctx, self.nstep, self.dt, self.rec, dobs, gl, gm,
+                                                misfit_out=mis)
+        self.torch.cuda.synchronize()
+        return gl.cpu().numpy(), gm.cpu().numpy(), float(mis.item())
+
+
+@pytest.mark.timeout(900)
+def test_config2_device_pointers_against_oracle():
+    mesh, true_cfg, start_cfg, rec_eq, src_eq = _config2_setup()
+    # 250 steps of a 60 Hz Ricker: the pulse has passed and its reflections from the
+    # anomaly reach the receivers (15 Hz would still be rising at 0.025 s)
+    nstep, dt, fr = 250, true_cfg["dt"], 60.0
+    print(f"config2 device-pointer case neq={mesh.neq} nstep={nstep} nrec={len(rec_eq)}", flush=True)
+    ds = _DeviceSolve(mesh, rec_eq, src_eq, nstep, dt, fr)
+    ds.material(true_cfg["model"])
+    dobs = ds.forward()
+    ds.material(start_cfg["model"])
+    tr = ds.forward(history=S.HR_HISTORY_STORE).cpu().numpy()
+    gl, gm, mis = ds.gradient(dobs)
+    dobs_h = dobs.cpu().numpy()
+    ds.ctx.close()
+    print("config2 GPU done; oracle run", flush=True)
+    ref = _oracle_run(mesh, start_cfg["model"], src_eq, np.ones(1), rec_eq, nstep, dt, fr, dmeas=dobs_h)
+    assert np.abs(ref["gm"]).max() > 0 and ref["misfit"] > 0
+    assert rel_l2(tr, ref["traces"]) < 1e-10
+    assert rel_l2(gl, ref["gl"]) < 1e-10 and rel_l2(gm, ref["gm"]) < 1e-10
+    assert abs(mis - ref["misfit"]) <= 1e-12 * ref["misfit"]
+
+
+@pytest.mark.timeout(900)
+def test_config2_full_length_checkpointed_equals_stored():
+    mesh, true_cfg, start_cfg, rec_eq, src_eq = _config2_setup()
+    nstep, dt, fr = true_cfg["nstep"], true_cfg["dt"], true_cfg["fr"]
+    print(f"config2 full-length history case neq={mesh.neq} nstep={nstep}", flush=True)
+    ds = _DeviceSolve(mesh, rec_eq, src_eq, nstep, dt, fr)
+    ds.material(true_cfg["model"])
+    dobs = ds.forward()
+    ds.material(start_cfg["model"])
+    out = {}
+    budget = 4.0e9
+    ds.ctx.set_history_budget(budget)
+    for name, hist, seg in (("store", S.HR_HISTORY_STORE, 0), ("checkpoint", S.HR_HISTORY_CHECKPOINT, 64),
+                            ("revolve", S.HR_HISTORY_REVOLVE, 0)):
+        ds.forward(history=hist, segment=max(seg, 1))
+        out[name] = ds.gradient(dobs) + (ds.ctx.recompute_ratio(),)
+        print(f"config2 {name}: recompute ratio {out[name][3]:.3f}", flush=True)
+    L, ns, nbytes, ratio = ds.ctx.history_plan(nstep, S.HR_HISTORY_REVOLVE)
+    ds.ctx.close()
+    assert 0 < L < nstep and nbytes <= budget
+    assert abs(out["revolve"][3] - ratio) < 1e-12
+    assert out["store"][3] == 1.0 and out["checkpoint"][3] > 1.5
+    for name in ("checkpoint", "revolve"):
+        assert np.array_equal(out[name][0], out["store"][0]), name
+        assert np.array_equal(out[name][1], out["store"][1]), name
+        assert out[name][2] == out["store"][2]
+
+
+# ----------------------------------------------------------------------- config 3
+@pytest.mark.timeout(900)
+def test_config3_sixteen_shots_batched_single_and_oracle():
+    cfg = config3(nshot=16)
+    mesh, dt = cfg["mesh"], cfg["dt"]
+    nstep, nshot = 120, 16
+    rec_eq = receiver_dofs(mesh, cfg["rec_nodes"])
+    src_eq = np.array([mesh.dof(nd, 2) for nd in cfg["src_nodes"]], dtype=np.int64)
+    src_shot = np.arange(nshot, dtype=np.int32)
+    ftab = np.stack([S.time_table(nstep, dt, 2, fr) for fr in cfg["frs"]])
+    one = np.ones(nshot)
+    print(f"config3 case neq={mesh.neq} nshot={nshot} nstep={nstep} nrec={len(rec_eq)}", flush=True)
+    ctx = S.Context(mesh, cfg["true"])
+    dobs = S.explicit_rk4_petsc_pml_3d_shots(ctx, nstep, dt, ftab, src_shot, src_eq, one, rec_eq,
+                                             history=S.HR_HISTORY_NONE).copy()
+    ctx.set_material(cfg["start"])
+    trb = S.explicit_rk4_petsc_pml_3d_shots(ctx, nstep, dt, ftab, src_shot, src_eq, one, rec_eq,
+                                            history=S.HR_HISTORY_CHECKPOINT, segment=32).copy()
+    glb, gmb, jb = S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dobs)
+    gl1, gm1, j1 = np.zeros(mesh.nnode), np.zeros(mesh.nnode), 0.0
+    for q in range(nshot):
+        tr1 = S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, ftab[q], src_eq[q:q + 1], one[:1], rec_eq,
+                                          history=S.HR_HISTORY_STORE)
+        assert np.array_equal(tr1, trb[q]), q
+        gl, gm, j = S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dobs[q])
+        gl1 += gl
+        gm1 += gm
+        j1 += j
+    ctx.close()
+    assert rel_l2(glb, gl1) < 1e-13 and rel_l2(gmb, gm1) < 1e-13 and abs(jb - j1) <= 1e-13 * j1
+    print("config3 batched == single; oracle run of shot 5", flush=True)
+    q = 5
+    ref = _oracle_run(mesh, cfg["start"], src_eq[q:q + 1], one[:1], rec_eq, nstep, dt, float(cfg["frs"][q]))
+    assert rel_l2(trb[q], ref["traces"]) < 1e-10
+
+
+# ----------------------------------------------------------- shared receiver equation
+def test_receivers_sharing_an_equation_follow_add_values():
+    mesh = BoxMesh(8, 8, 6, npml=(2,) * 5)
+    model, true = random_model(mesh, seed=41), random_model(mesh, seed=42)
+    rec_eq = receiver_dofs(mesh, surface_grid_nodes(mesh, 3, 3, -2.0, 2.0, -2.0, 2.0))
+    rec_eq = np.concatenate([rec_eq, rec_eq[[4, 4]]])          # one equation three times
+    src_eq = np.array([mesh.dof(mesh.surface_node(0.0, 0.0), 2)], dtype=np.int64)
+    nstep, dt, fr = 160, 1e-4, 60.0
+    print(f"shared-equation receivers case nrec={len(rec_eq)}", flush=True)
+    dmeas = _oracle_run(mesh, true, src_eq, np.ones(1), rec_eq, nstep, dt, fr)["traces"]
+    dmeas[:, -1] *= 0.5                                         # the copies see different data
+    ref = _oracle_run(mesh, model, src_eq, np.ones(1), rec_eq, nstep, dt, fr, dmeas=dmeas)
+    ctx = S.Context(mesh, model)
+    S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, S.time_table(nstep, dt, 2, fr), src_eq, np.ones(1), rec_eq)
+    gl, gm, mis = S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dmeas)
+    assert abs(mis - ref["misfit"]) <= 1e-12 * ref["misfit"]
+    assert abs(S.compute_misfit(ctx, nstep, dt, dmeas) - ref["misfit"]) <= 1e-12 * ref["misfit"]
+    assert rel_l2(gl, ref["gl"]) < 1e-10 and rel_l2(gm, ref["gm"]) < 1e-10
+
+
+# --------------------------------------------------------------------- history
+def test_revolve_small_budget_equals_store_and_plan():
+    mesh = BoxMesh(9, 8, 7, npml=(2,) * 5)
+    model, true = random_model(mesh, seed=51), random_model(mesh, seed=52)
+    rec_eq = receiver_dofs(mesh, surface_grid_nodes(mesh, 3, 3, -2.0, 2.0, -2.0, 2.0))
+    src_eq = np.array([mesh.dof(mesh.surface_node(0.0, 0.0), 2)], dtype=np.int64)
+    nstep, dt, fr = 300, 1e-4, 60.0
+    dmeas = _oracle_run(mesh, true, src_eq, np.ones(1), rec_eq, nstep, dt, fr)["traces"]
+    ftab = S.time_table(nstep, dt, 2, fr)
+    ctx = S.Context(mesh, model)
+    S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, ftab, src_eq, np.ones(1), rec_eq, history=S.HR_HISTORY_STORE)
+    ref = S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dmeas)
+    # budgets from a few snapshots down to none: slices of one segment + s snapshots
+    nbox_slice = 3 * 8 * ctx.nstore // 3
+    for nsnap_target in (5, 2, 0):
+        budget = 9 * nbox_slice + nsnap_target * 3 * 8 * (9 * mesh.nnode + (mesh.nnode & 1)) + 1
+        ctx.set_history_budget(budget)
+        L, ns, nbytes, ratio = ctx.history_plan(nstep, S.HR_HISTORY_REVOLVE)
+        print(f"revolve case budget={budget} -> segment {L}, snapshots {ns}, ratio {ratio:.3f}", flush=True)
+        assert 0 < L < nstep and nbytes <= budget
+        S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, ftab, src_eq, np.ones(1), rec_eq,
+                                    history=S.HR_HISTORY_REVOLVE)
+        out = S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dmeas)
+        assert abs(ctx.recompute_ratio() - ratio) < 1e-12
+        assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1]) and out[2] == ref[2]
+    # a budget that holds every slice keeps them all
+    ctx.set_history_budget(1e12)
+    assert ctx.history_plan(nstep, S.HR_HISTORY_REVOLVE)[0] == 0
+
+
+def test_set_material_invalidates_the_history():
+    mesh = BoxMesh(6, 6, 5, npml=(2,) * 5)
+    model = random_model(mesh, seed=61)
+    rec_eq = receiver_dofs(mesh, surface_grid_nodes(mesh, 2, 2, -1.0, 1.0, -1.0, 1.0))
+    src_eq = np.array([mesh.dof(mesh.surface_node(0.0, 0.0), 2)], dtype=np.int64)
+    nstep, dt = 30, 1e-4
+    ctx = S.Context(mesh, model)
+    S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, S.time_table(nstep, dt, 2, 60.0), src_eq, np.ones(1), rec_eq)
+    ctx.set_material(random_model(mesh, seed=62))
+    with pytest.raises(S.HrError):
+        S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, np.zeros((nstep + 1, len(rec_eq))))

@@ -113,6 +113,13 @@ def test_cabi_library_exports_every_declared_symbol():
 def test_host_time_function_matches_oracle():
     from hirensi_amd import solver as S
     from oracle import oracle as O
+    cases = [(2, 15.0), (2, 60.0), (3, 0.0), (1, 5.0), (6, 0.0), (7, 15.0), (7, 40.0), (8, 0.0), (9, 0.0), (10, 0.0)]
     for t in np.linspace(0, 0.2, 37):
-        for lc, fr in ((2, 15.0), (2, 60.0), (3, 0.0), (1, 5.0)):
-            assert S.time_function(t, lc, fr) == O.time_function(t, lc, fr)
+        for lc, fr in cases:
+            assert S.time_function(t, lc, fr) == O.time_function(t, lc, fr), (t, lc)
+    # Chopra's cosine wave over its 10 s duration and past it
+    for t in np.linspace(0, 10.5, 211):
+        for lc in (4, 5):
+            assert S.time_function(t, lc, 0.0) == O.time_function(t, lc, 0.0), (t, lc)
+    with pytest.raises(S.HrError):
+        S.time_function(0.1, 11, 15.0)
