@@ -13,6 +13,7 @@
 #include <unordered_map>
 #include "hr_device.h"
 #include "hr_revolve.h"
+#include "hr_internal.h"
 #include "../../include/hirensi.h"
 
 #pragma clang fp contract(off)
@@ -48,6 +49,7 @@ struct hr_ctx {
   std::string err;
   int flags = 0;
   Geo g{};
+  double hel[3] = {1.0, 1.0, 1.0};          // element edges
   long long neq = 0;
   std::vector<long long> eq_slot;          // host copy of the numbering
   long long* d_eq_slot = nullptr;
@@ -98,6 +100,13 @@ struct hr_ctx {
   int* d_mis_idx = nullptr;
   // staging of host vectors for hr_apply (neq each)
   double *d_ax = nullptr, *d_ay = nullptr;
+  // regular-domain lumped mass per node (DIAG_M_RD of each displacement DOF)
+  double* d_mrd = nullptr;
+  // regular-domain energy every e_every steps of the forward (0: off)
+  int e_every = 0, e_rows = 0;
+  double *d_energy = nullptr, *d_epart = nullptr;
+  // one history slice for hr_gradient_accumulate
+  double* d_uslice = nullptr;
   // capacities of the grow-only device allocations, by the address of their pointer field
   std::unordered_map<void*, size_t> caps;
   // forward sources for recomputation
@@ -673,6 +682,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
     return fail(ctx, 1, "PML layers leave no regular domain");
   if (!(d->h[0] > 0 && d->h[1] > 0 && d->h[2] > 0)) return fail(ctx, 1, "element edges must be positive");
   ctx->flags = d->flags;
+  for (int a = 0; a < 3; ++a) ctx->hel[a] = d->h[a];
   Geo& g = ctx->g;
   g.nex = d->nex; g.ney = d->ney; g.nez = d->nez;
   g.NX = 2 * g.nex + 1; g.NY = 2 * g.ney + 1; g.NZ = 2 * g.nez + 1;
@@ -756,6 +766,7 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   for (double* p : ctx->snap) F(p);
   for (int a = 0; a < 6; ++a) F(ctx->adj_buf[a]);
   F(ctx->d_mis_ptr); F(ctx->d_mis_idx); F(ctx->d_ax); F(ctx->d_ay);
+  F(ctx->d_mrd); F(ctx->d_energy); F(ctx->d_epart); F(ctx->d_uslice);
   F(ctx->d_src_tile_ptr); F(ctx->d_src_node); F(ctx->d_src_comp); F(ctx->d_src_m); F(ctx->d_src_val);
   F(ctx->d_rec_tile_ptr); F(ctx->d_rec_node); F(ctx->d_rec_comp); F(ctx->d_rec_m); F(ctx->d_rec_perm);
   F(ctx->d_obs); F(ctx->d_gl); F(ctx->d_gm); F(ctx->d_part); F(ctx->d_scalar); F(ctx->d_tr_sorted);
@@ -887,6 +898,7 @@ int hr_set_material(hr_ctx* ctx, const double* lam, const double* mu, const doub
     for (int k = 0; k < g.NZ; ++k) { double s[6]; h_stretch(g.pml, xc[0], yc[0], zc[k], s); az[2 * k] = s[2]; az[2 * k + 1] = s[5]; }
     if (int rc = upload(ctx, &ctx->d_tab, tb)) return rc;
   }
+  if (int rc = upload(ctx, &ctx->d_mrd, mrd)) return rc;
   ctx->have_material = true;
   // the history and traces of an earlier forward belong to the old operator
   ctx->f_nstep = 0;
@@ -1261,6 +1273,7 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
   }
   if (int rc = zero_state(ctx)) return rc;
   if (scheme == 1) {
+    ctx->e_rows = 0;
     // traces row n = u at the start of step n (n >= 1; row 0 is u_0 = 0), as U_Store(:, ISTEP)
     if (int rc = record_step(ctx, 0)) return rc;
     for (int n = 1; n <= nstep; ++n) {
@@ -1286,9 +1299,23 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
   }
   const int last0 = L > 0 ? (ctx->h_nseg - 1) * L : 0;
   const size_t seg_rows = (size_t)L + 1;
+  // regular-domain energy of every shot at steps 0, e_every, 2 e_every, ... <= nstep
+  ctx->e_rows = ctx->e_every > 0 ? nstep / ctx->e_every + 1 : 0;
+  if (ctx->e_rows) {
+    if (int rc = dalloc(ctx, &ctx->d_energy, (size_t)nshot * ctx->e_rows)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_epart, (size_t)energy_partials_size())) return rc;
+  }
+  auto energy_at = [ctx, nshot](int n) -> int {
+    const size_t nf = field_size(ctx);
+    for (int q = 0; q < nshot; ++q)
+      HIPC(launch_energy(ctx->g.NN, ctx->d_mrd, ctx->x[2] + q * nf, ctx->d_epart,
+                         ctx->d_energy + (size_t)q * ctx->e_rows + n / ctx->e_every, ctx->stream));
+    return 0;
+  };
   for (int n = 0; n < nstep; ++n) {
     // traces and history slice of the state at the start of step n, queued beside its stage 1
     ctx->deferred.push_back([ctx, n]() { return record_step(ctx, n); });
+    if (ctx->e_rows && n % ctx->e_every == 0) ctx->deferred.push_back([energy_at, n]() { return energy_at(n); });
     if (ctx->h_store)
       ctx->deferred.push_back([ctx, rows, n]() { return store_slice(ctx, ctx->d_hist, rows, (size_t)n); });
     if (L > 0) {
@@ -1305,6 +1332,7 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
   }
   if (L > 0) if (int rc = store_slice(ctx, ctx->d_seg, seg_rows, (size_t)(nstep - last0))) return rc;
   if (int rc = record_step(ctx, nstep)) return rc;
+  if (ctx->e_rows && nstep % ctx->e_every == 0) if (int rc = energy_at(nstep)) return rc;
   if (ctx->h_store) if (int rc = store_slice(ctx, ctx->d_hist, rows, (size_t)nstep)) return rc;
   if (traces && nrec) {
     HIPC(hipMemcpyAsync(traces, ctx->d_traces, (size_t)nshot * rows * nrec * sizeof(double),
@@ -1347,29 +1375,47 @@ int hr_forward(hr_ctx* ctx, int nstep, double dt, const double* ftab, int nsrc, 
 
 }  // extern "C"
 
-namespace {
-// misfit of every shot of the last forward against observations on the device, summed over
-// shots in shot order into *out (device)
-int misfit_all(hr_ctx* ctx, double dt, const double* dobs, double* out) {
+namespace hr {
+size_t misfit_scratch_size(const hr_ctx* ctx) { return (size_t)ctx->f_nshot * ((size_t)ctx->f_nstep + 1) + ctx->f_nshot; }
+
+int misfit_check(hr_ctx* ctx, int nstep, int nrec, const void* dis_meas, const void* misfit) {
+  if (int rc = check_ready(ctx)) return rc;
+  if (nstep != ctx->f_nstep || nrec != ctx->f_nrec) return fail(ctx, 1, "nstep/nrec differ from the last forward");
+  if (!dis_meas || !misfit) return fail(ctx, 1, "null argument");
+  return 0;
+}
+
+// Compute_Misfit of every shot of the last forward against observations on the
+// device, summed over shots in shot order into *out (device); part holds
+// misfit_scratch_size(ctx) doubles, all written before they are read
+int misfit_launch(hr_ctx* ctx, double dt, const double* dobs, double* part, double* out) {
   const int nshot = ctx->f_nshot, nstep = ctx->f_nstep, nrec = ctx->f_nrec;
   const size_t rows = (size_t)nstep + 1;
-  if (int rc = dalloc(ctx, &ctx->d_part, (size_t)nshot * rows + nshot)) return rc;
-  double* per_shot = ctx->d_part + (size_t)nshot * rows;
+  double* per_shot = part + (size_t)nshot * rows;
   for (int q = 0; q < nshot; ++q)
     HIPC(launch_misfit(nstep, ctx->mis_ngrp, ctx->d_mis_ptr, ctx->d_mis_idx, nrec,
                        ctx->d_traces + (size_t)q * rows * nrec, dobs + (size_t)q * rows * nrec, dt,
-                       ctx->d_part + (size_t)q * rows, per_shot + q, ctx->stream));
+                       part + (size_t)q * rows, per_shot + q, ctx->stream));
   HIPC(launch_sum(nshot, per_shot, out, ctx->stream));
   return 0;
+}
+
+bool ctx_device_ptrs(const hr_ctx* ctx) { return ctx->flags & HR_DEVICE_PTRS; }
+hipStream_t ctx_stream(const hr_ctx* ctx) { return ctx->stream; }
+int ctx_fail(hr_ctx* ctx, int code, const char* msg) { return fail(ctx, code, "%s", msg); }
+}  // namespace hr
+
+namespace {
+int misfit_all(hr_ctx* ctx, double dt, const double* dobs, double* out) {
+  if (int rc = dalloc(ctx, &ctx->d_part, hr::misfit_scratch_size(ctx))) return rc;
+  return hr::misfit_launch(ctx, dt, dobs, ctx->d_part, out);
 }
 }  // namespace
 
 extern "C" {
 
 int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_meas, double* misfit) {
-  if (int rc = check_ready(ctx)) return rc;
-  if (nstep != ctx->f_nstep || nrec != ctx->f_nrec) return fail(ctx, 1, "nstep/nrec differ from the last forward");
-  if (!dis_meas || !misfit) return fail(ctx, 1, "null argument");
+  if (int rc = hr::misfit_check(ctx, nstep, nrec, dis_meas, misfit)) return rc;
   const bool dev = ctx->flags & HR_DEVICE_PTRS;
   const size_t nobs = (size_t)ctx->f_nshot * (nstep + 1) * nrec;
   const double* dobs = dis_meas;
@@ -1630,6 +1676,146 @@ int hr_get_history_plan(hr_ctx* ctx, int nstep, int nshot, int history, int segm
   ctx->h_nsnap = s0;
   ctx->h_plan = p0;
   return rc;
+}
+
+int hr_set_energy_interval(hr_ctx* ctx, int every) {
+  if (!ctx) return 1;
+  if (every < 0) return fail(ctx, 1, "energy interval must be >= 0");
+  ctx->e_every = every;
+  return 0;
+}
+
+int hr_get_energy(hr_ctx* ctx, double* energy, int* nrow) {
+  if (!ctx || !nrow) return fail(ctx, 1, "null argument");
+  *nrow = ctx->e_rows;
+  if (!energy || ctx->e_rows == 0) return 0;
+  const size_t n = (size_t)ctx->f_nshot * ctx->e_rows;
+  HIPC(hipMemcpyAsync(energy, ctx->d_energy, n * sizeof(double),
+                      (ctx->flags & HR_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int hr_gradient_accumulate(hr_ctx* ctx, const double* u, const double* p, double scale, double* g_lambda,
+                           double* g_mu) {
+  if (int rc = check_ready(ctx)) return rc;
+  if (!u || !p || !g_lambda || !g_mu) return fail(ctx, 1, "null argument");
+  const bool dev = ctx->flags & HR_DEVICE_PTRS;
+  const long long neq = ctx->neq;
+  const size_t NN = (size_t)ctx->g.NN;
+  const double *du = u, *dp = p;
+  if (!dev) {
+    HIPC(hipMemcpyAsync(ctx->d_ax, u, neq * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->d_ay, p, neq * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    du = ctx->d_ax;
+    dp = ctx->d_ay;
+  }
+  // u and p on the lattice (the scratch fields hold zeros at inactive slots), u's
+  // regular-domain displacement as one history slice
+  HIPC(launch_gather_eq(neq, ctx->d_eq_slot, du, nullptr, ctx->tmp, ctx->stream));
+  HIPC(launch_gather_eq(neq, ctx->d_eq_slot, dp, nullptr, ctx->tmp2, ctx->stream));
+  if (int rc = dalloc(ctx, &ctx->d_uslice, 3 * (size_t)ctx->nbox)) return rc;
+  if (int rc = dalloc(ctx, &ctx->d_gl, ctx->nbox)) return rc;
+  if (int rc = dalloc(ctx, &ctx->d_gm, ctx->nbox)) return rc;
+  HIPC(launch_store_box(ctx->g, ctx->box0, ctx->boxn, ctx->tmp, ctx->d_uslice, ctx->stream));
+  HIPC(hipMemsetAsync(ctx->d_gl, 0, ctx->nbox * sizeof(double), ctx->stream));
+  HIPC(hipMemsetAsync(ctx->d_gm, 0, ctx->nbox * sizeof(double), ctx->stream));
+  HIPC(launch_gradient(ctx->g, ctx->box0, ctx->boxn, ctx->d_uslice, ctx->tmp2, scale, ctx->d_gl, ctx->d_gm,
+                       ctx->stream));
+  double *gl = g_lambda, *gm = g_mu;
+  if (!dev) {
+    if (int rc = dalloc(ctx, &ctx->d_pg, 2 * NN)) return rc;
+    gl = ctx->d_pg;
+    gm = ctx->d_pg + NN;
+    HIPC(hipMemcpyAsync(gl, g_lambda, NN * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(gm, g_mu, NN * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  HIPC(launch_box_add(ctx->g, ctx->box0, ctx->boxn, ctx->d_gl, ctx->d_gm, gl, gm, ctx->stream));
+  if (!dev) {
+    HIPC(hipMemcpyAsync(g_lambda, gl, NN * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipMemcpyAsync(g_mu, gm, NN * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int hr_set_numbering(hr_ctx* ctx, const long long* id) {
+  if (!ctx || !id) return fail(ctx, 1, "null argument");
+  const Geo& g = ctx->g;
+  const size_t n9 = (size_t)NCOMP * (size_t)g.NN;
+  std::vector<long long> h(n9);
+  if (ctx->flags & HR_DEVICE_PTRS) HIPC(hipMemcpy(h.data(), id, n9 * sizeof(long long), hipMemcpyDeviceToHost));
+  else std::memcpy(h.data(), id, n9 * sizeof(long long));
+  std::vector<long long> slot((size_t)ctx->neq, -1);
+  for (int k = 0; k < g.NZ; ++k)
+    for (int j = 0; j < g.NY; ++j)
+      for (int i = 0; i < g.NX; ++i) {
+        const long long n = lat(g, i, j, k);
+        for (int c = 0; c < NCOMP; ++c) {
+          const long long e = h[(size_t)NCOMP * n + c];
+          const bool act = h_active(g, i, j, k, c);
+          if (act != (e >= 0))
+            return fail(ctx, 1, "numbering: component %d of node %lld is %s here", c, n, act ? "active" : "constrained");
+          if (!act) continue;
+          if (e >= ctx->neq || slot[e] >= 0) return fail(ctx, 1, "numbering: equation %lld out of range or repeated", e);
+          slot[e] = slot_of(g, c, n);
+        }
+      }
+  ctx->eq_slot.swap(slot);
+  if (int rc = upload(ctx, &ctx->d_eq_slot, ctx->eq_slot)) return rc;
+  // MatShell scalings and the last forward were expressed in the old numbering
+  for (int w = 0; w < 3; ++w) {
+    ctx->shell[w].s = 1.0;
+    dfree(ctx, &ctx->shell[w].l);
+    dfree(ctx, &ctx->shell[w].r);
+  }
+  ctx->f_nstep = 0;
+  ctx->f_history = HR_HISTORY_NONE;
+  return 0;
+}
+
+int hr_surface_force(hr_ctx* ctx, const int* nface, long long* eq, double* val, long long* nnz) {
+  if (!ctx || !nface || !nnz) return fail(ctx, 1, "null argument");
+  const Geo& g = ctx->g;
+  // LFACE, FVAL, IPERM of ELEMENT_FS_3D: the face is xi_N = FVAL with in-face axes L, M
+  const int LFACE[6] = {2, 2, 3, 3, 1, 1};
+  const double FVAL[6] = {+1.0, -1.0, +1.0, -1.0, +1.0, -1.0};
+  const int IPERM[3] = {2, 3, 1};
+  const double w[3] = {1.0 / 3.0, 4.0 / 3.0, 1.0 / 3.0};
+  const double* h = ctx->hel;
+  std::vector<double> B((size_t)ctx->neq, 0.0);
+  std::vector<long long> inv(field_size(ctx), -1);
+  for (long long e = 0; e < ctx->neq; ++e) inv[ctx->eq_slot[e]] = e;
+  for (int ez = 0; ez < g.nez; ++ez)
+    for (int ey = 0; ey < g.ney; ++ey)
+      for (int ex = 0; ex < g.nex; ++ex) {
+        const bool rd = ex >= g.rlo[0] && ex < g.rhi[0] && ey >= g.rlo[1] && ey < g.rhi[1] && ez >= g.rlo[2];
+        int f = nface[(size_t)ex + (size_t)g.nex * ((size_t)ey + (size_t)g.ney * ez)];
+        if (!rd || f == 0) continue;                  // MTYPE 1 only; NFACE 0: no traction
+        if (f < 0 || f > 6) return fail(ctx, 1, "face code %d of element (%d, %d, %d) not in 0..6", f, ex, ey, ez);
+        if (f == 6) f = 5;                            // as ELEMENT_FS_3D maps it
+        const int L = LFACE[f - 1], M = IPERM[L - 1], N = IPERM[M - 1];
+        // affine element: |d x / d xi_L  x  d x / d xi_M| = h_L h_M / 4
+        const double detj_surf = (0.5 * h[L - 1]) * (0.5 * h[M - 1]);
+        for (int lj = 0; lj < 3; ++lj)
+          for (int li = 0; li < 3; ++li) {
+            int loc[3];
+            loc[N - 1] = FVAL[f - 1] > 0.0 ? 2 : 0;
+            loc[L - 1] = li;
+            loc[M - 1] = lj;
+            const long long n = lat(g, 2 * ex + loc[0], 2 * ey + loc[1], 2 * ez + loc[2]);
+            const long long q = inv[slot_of(g, 2, n)];  // unit traction in z
+            if (q >= 0) B[q] += (1.0 * 1.0) * ((w[li] * w[lj]) * detj_surf);
+          }
+      }
+  long long k = 0;
+  for (long long q = 0; q < ctx->neq; ++q)
+    if (B[q] != 0.0) {
+      if (eq && val) { eq[k] = q; val[k] = B[q]; }
+      ++k;
+    }
+  *nnz = k;
+  return 0;
 }
 
 int hr_revolve_schedule(int nseg, int nsnap, int max_actions, int* actions, int* nactions, long long* advances) {

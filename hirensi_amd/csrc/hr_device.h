@@ -54,6 +54,12 @@ hipError_t launch_gradient_finish(const Geo& g, const int* box0, const int* boxn
                                   const double* gmb, double* gl, double* gm, hipStream_t s);
 hipError_t launch_permute_cols(long long nrows, int ncols, const long long* perm, const double* src, double* dst,
                                hipStream_t s);
+// g += box sums on the regular-domain node box
+hipError_t launch_box_add(const Geo& g, const int* box0, const int* boxn, const double* glb, const double* gmb,
+                          double* gl, double* gm, hipStream_t s);
+// out = 0.5 sum_n mrd[n] |x3 of node n|^2 (displacement block); part: energy_partials_size() doubles
+int energy_partials_size();
+hipError_t launch_energy(long long nn, const double* mrd, const double* x3, double* part, double* out, hipStream_t s);
 hipError_t launch_sum(int n, const double* in, double* out, hipStream_t s);
 // receivers grouped by equation: group g = receivers gidx[gptr[g] .. gptr[g+1])
 hipError_t launch_misfit(int nstep, int ngrp, const int* gptr, const int* gidx, int nrec, const double* tr,

@@ -206,6 +206,51 @@ __global__ void gradient_finish_kernel(Geo g, int bi0, int bj0, int bk0, int bnx
   }
 }
 
+// g[n] += box[r] over the regular-domain node box (one Update_g step into the
+// caller's nodal gradients)
+__global__ void box_add_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int bny, int bnz,
+                               const double* __restrict__ gl_box, const double* __restrict__ gm_box,
+                               double* __restrict__ gl, double* __restrict__ gm) {
+  const long long nbox = (long long)bnx * bny * bnz;
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < nbox; r += (long long)gridDim.x * blockDim.x) {
+    const int bi = (int)(r % bnx), bj = (int)((r / bnx) % bny), bk = (int)(r / ((long long)bnx * bny));
+    const long long n = (long long)(bi0 + bi) + (long long)g.NX * ((long long)(bj0 + bj) + (long long)g.NY * (bk0 + bk));
+    gl[n] = gl[n] + gl_box[r];
+    gm[n] = gm[n] + gm_box[r];
+  }
+}
+
+// Regular-domain energy (TOTAL_ENERGY_RD_PETSC_EXPLICIT, Obsolete.F90:108-150):
+// 0.5 (u . AK_RD u + (M_RD x3) . x3); AK_RD is never assembled (the insertion is
+// commented out, Assemble_PETSc_3D.F90:163), so only the kinetic term remains.
+// Fixed grid of ENERGY_BLOCKS blocks, partial sums in block order: deterministic.
+constexpr int ENERGY_BLOCKS = 256;
+__global__ void __launch_bounds__(256) energy_partial_kernel(long long nn, const double* __restrict__ mrd,
+                                                             const double* __restrict__ x3,
+                                                             double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long long n = blockIdx.x * 256LL + threadIdx.x; n < nn; n += 256LL * gridDim.x) {
+    const double m = mrd[n];
+    const double a = x3[3 * n], b = x3[3 * n + 1], c = x3[3 * n + 2];
+    acc += (m * a) * a + (m * b) * b + (m * c) * c;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void energy_finish_kernel(const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double c = 0.0;
+    for (int b = 0; b < ENERGY_BLOCKS; ++b) c += part[b];
+    out[0] = 0.50 * (0.0 + c);
+  }
+}
+
 }  // namespace hr
 
 // ------------------------------------------------------------ launchers
@@ -250,6 +295,19 @@ hipError_t launch_gradient_finish(const Geo& g, const int* b0, const int* bn, co
   const long long tot = (long long)bn[0] * bn[1] * bn[2];
   hipLaunchKernelGGL(gradient_finish_kernel, dim3(grid_for(tot)), dim3(256), 0, s, g, b0[0], b0[1], b0[2], bn[0],
                      bn[1], bn[2], glb, gmb, gl, gm);
+  return hipGetLastError();
+}
+hipError_t launch_box_add(const Geo& g, const int* b0, const int* bn, const double* glb, const double* gmb, double* gl,
+                          double* gm, hipStream_t s) {
+  const long long tot = (long long)bn[0] * bn[1] * bn[2];
+  hipLaunchKernelGGL(box_add_kernel, dim3(grid_for(tot)), dim3(256), 0, s, g, b0[0], b0[1], b0[2], bn[0], bn[1],
+                     bn[2], glb, gmb, gl, gm);
+  return hipGetLastError();
+}
+int energy_partials_size() { return ENERGY_BLOCKS; }
+hipError_t launch_energy(long long nn, const double* mrd, const double* x3, double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(energy_partial_kernel, dim3(ENERGY_BLOCKS), dim3(256), 0, s, nn, mrd, x3, part);
+  hipLaunchKernelGGL(energy_finish_kernel, dim3(1), dim3(64), 0, s, part, out);
   return hipGetLastError();
 }
 hipError_t launch_permute_cols(long long nrows, int ncols, const long long* perm, const double* src, double* dst,

@@ -150,6 +150,21 @@ class BoxMesh:
         if np.any(self.store_eq < 0):
             raise ValueError("stored RD displacement DOF is constrained")
 
+    def petsc_numbering(self, node_order):
+        """ID table of the equation numbering the reference's partitioner builds
+        (PIC_ReNumbering.F90:263-285): nodes taken in ``node_order`` (PETSc node
+        order, e.g. grouped by partition), the active DOFs of each node numbered
+        consecutively.  Returns (nnode, 9) in lattice order, -1 where constrained."""
+        node_order = np.asarray(node_order, dtype=np.int64)
+        if node_order.shape != (self.nnode,) or np.any(np.sort(node_order) != np.arange(self.nnode)):
+            raise ValueError("node_order must be a permutation of the lattice nodes")
+        act = self.active[node_order]                     # PETSc node-major
+        eq = np.full(act.shape, -1, dtype=np.int64)
+        eq[act] = np.arange(int(act.sum()), dtype=np.int64)
+        out = np.full((self.nnode, NDOF), -1, dtype=np.int64)
+        out[node_order] = eq
+        return out
+
     # ------------------------------------------------------------- reference data
     def xyz(self):
         X, Y, Z = np.meshgrid(self.xc, self.yc, self.zc, indexing="ij")

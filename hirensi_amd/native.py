@@ -23,6 +23,7 @@ EXPORTS = (
     "hr_misfit", "hr_recompute_ratio", "hr_set_tiling", "hr_set_profiling", "hr_get_kernel_stats",
     "hr_forward_shots", "hr_gradient_mass", "hr_regularization", "hr_regularize_gradient",
     "hr_forward_cd", "hr_set_history_budget", "hr_get_history_plan", "hr_revolve_schedule",
+    "hr_gradient_accumulate", "hr_set_numbering", "hr_surface_force", "hr_set_energy_interval", "hr_get_energy",
 )
 
 
@@ -100,6 +101,11 @@ def load():
     L.hr_get_history_plan.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.hr_revolve_schedule.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+    L.hr_gradient_accumulate.argtypes = [_vp, _dp, _dp, C.c_double, _dp, _dp]
+    L.hr_set_numbering.argtypes = [_vp, _dp]
+    L.hr_surface_force.argtypes = [_vp, _dp, _dp, _dp, C.POINTER(C.c_longlong)]
+    L.hr_set_energy_interval.argtypes = [_vp, C.c_int]
+    L.hr_get_energy.argtypes = [_vp, _dp, C.POINTER(C.c_int)]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     _lib = L

@@ -87,6 +87,7 @@ class Context:
             d.h[a] = mesh.h[a]
         d.top_stress_bc = 1 if mesh.top_stress_bc == "reference" else 0
         d.flags = native.HR_DEVICE_PTRS if device_ptrs else 0
+        self._last_nshot = 1
         h = C.c_void_p()
         rc = self.L.hr_ctx_create(C.byref(d), C.byref(h))
         self.h = h
@@ -150,6 +151,53 @@ class Context:
 
     def recompute_ratio(self):
         return self.L.hr_recompute_ratio(self.h)
+
+    def set_numbering(self, ID):
+        """Equation numbering of the caller (``hr_set_numbering``): ``ID`` is
+        (nnode, 9) in lattice node order, the equation number of each DOF or -1
+        where constrained (the reference's ID_PETSc mapped onto the lattice)."""
+        ID = np.ascontiguousarray(ID, dtype=np.int64)
+        if ID.shape != (self.nnode, 9):
+            raise ValueError("ID must be (nnode, 9)")
+        if self.device_ptrs:
+            raise HrError("set_numbering takes a host array here; pass device memory to hr_set_numbering directly")
+        self._check(self.L.hr_set_numbering(self.h, ID.ctypes.data))
+
+    def surface_force(self, nface):
+        """ASSEM_FORCE_PETSC_3D: the (eq, val) entries of B for a unit z traction
+        on face ``nface[e]`` (0 none, 1..6) of each regular-domain element."""
+        nface = np.ascontiguousarray(nface, dtype=np.int32)
+        n = C.c_longlong()
+        self._check(self.L.hr_surface_force(self.h, nface.ctypes.data, None, None, C.byref(n)))
+        eq = np.zeros(n.value, dtype=np.int64)
+        val = np.zeros(n.value)
+        self._check(self.L.hr_surface_force(self.h, nface.ctypes.data, eq.ctypes.data, val.ctypes.data, C.byref(n)))
+        return eq, val
+
+    def set_energy_interval(self, every):
+        """Record the regular-domain energy every ``every`` RK4 steps (0: off)."""
+        self._check(self.L.hr_set_energy_interval(self.h, int(every)))
+
+    def energy(self):
+        """Energies of the last forward, (nshot, nrow) (host contexts)."""
+        n = C.c_int()
+        self._check(self.L.hr_get_energy(self.h, None, C.byref(n)))
+        if self.device_ptrs:
+            raise HrError("energy returns host data; use hr_get_energy with a device buffer")
+        out = np.zeros(max(n.value, 0) * self._last_nshot)
+        self._check(self.L.hr_get_energy(self.h, out.ctypes.data if out.size else None, C.byref(n)))
+        return out.reshape(self._last_nshot, n.value)
+
+    def gradient_accumulate(self, u, p, scale, g_lambda, g_mu):
+        """Update_g: g += scale * (lambda, mu) element gradients of u and p
+        (neq vectors), in place on the nodal g_lambda, g_mu."""
+        if not self.device_ptrs:
+            u, p = _f64(u), _f64(p)
+            for a in (g_lambda, g_mu):
+                if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]):
+                    raise ValueError("g_lambda and g_mu must be contiguous float64 arrays (updated in place)")
+        self._check(self.L.hr_gradient_accumulate(self.h, _ptr(u), _ptr(p), float(scale), _ptr(g_lambda),
+                                                  _ptr(g_mu)))
 
     def set_history_budget(self, nbytes):
         """HBM bytes for the HR_HISTORY_REVOLVE history (0: half the free memory)."""
@@ -265,6 +313,7 @@ def explicit_rk4_petsc_pml_3d(ctx: Context, nstep, dt, ftab, src_eq, src_val, re
         if traces is None:
             traces = np.zeros((nstep + 1, nrec))
         fptr, sptr, vptr, rptr = ftab.ctypes.data, src_eq.ctypes.data, src_val.ctypes.data, rec_eq.ctypes.data
+    ctx._last_nshot = 1
     ctx._check(ctx.L.hr_forward(ctx.h, int(nstep), float(dt), fptr, nsrc, sptr if nsrc else None,
                                 vptr if nsrc else None, nrec, rptr if nrec else None,
                                 _ptr(traces) if nrec else None, int(history), int(segment)))
@@ -296,6 +345,7 @@ def explicit_rk4_petsc_pml_3d_shots(ctx: Context, nstep, dt, ftab, src_shot, src
             traces = np.zeros((nshot, nstep + 1, nrec))
         ptrs = (ftab.ctypes.data, src_shot.ctypes.data, src_eq.ctypes.data, src_val.ctypes.data,
                 rec_eq.ctypes.data)
+    ctx._last_nshot = int(nshot)
     ctx._check(ctx.L.hr_forward_shots(ctx.h, int(nshot), int(nstep), float(dt), ptrs[0], nsrc,
                                       ptrs[1] if nsrc else None, ptrs[2] if nsrc else None,
                                       ptrs[3] if nsrc else None, nrec, ptrs[4] if nrec else None,

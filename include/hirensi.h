@@ -143,6 +143,44 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
 /* Compute_Misfit of the traces of the last forward against dis_meas (summed over its shots). */
 int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_meas, double* misfit);
 
+/* Update_g (Assemble_PETSc_3D.F90:556-661) for one step of the control pass
+ * (Solver_PETSc_Explicit.F90:1474-1510): over the regular-domain elements
+ * (MTYPE 1), g_lambda += scale * W div u div p and g_mu += scale * W 2 eps(u):eps(p)
+ * at every GLL point, i.e. ELEMENT_g_3D assembled with ADD_VALUES and
+ * multiplied by Scale_L.  u, p: neq entries in the context's numbering (the
+ * displacement DOFs of the regular-domain nodes are read, as the reference
+ * scatters only U_Store/P_Store entries); g_lambda, g_mu: nodal, accumulated. */
+int hr_gradient_accumulate(hr_ctx* ctx, const double* u, const double* p, double scale, double* g_lambda,
+                           double* g_mu);
+
+/* Caller-supplied equation numbering (replaces the lattice order): id[9*n + c]
+ * = equation number of component c (u_x u_y u_z s_xx s_yy s_zz s_xy s_xz s_yz)
+ * of lattice node n, -1 where constrained -- the reference's ID_PETSc from
+ * PIC_ReNumbering.F90:263-285 (nodes in PETSc order, active DOFs interleaved)
+ * mapped onto the lattice.  The active set must be the context's; the numbers
+ * a permutation of 0..neq-1.  Every later call that takes equation numbers or
+ * neq-vectors uses it; resets the MatShell scalings and the last forward. */
+int hr_set_numbering(hr_ctx* ctx, const long long* id);
+
+/* Surface traction force (ASSEM_FORCE_PETSC_3D, Assemble_PETSc_3D.F90:444-548,
+ * with ELEMENT_FS_3D, Element_Matrices_3D.F90:890-1096): a unit traction in z on
+ * face nface[e] of every regular-domain element (ID_BC(IEL,1): 0 none, 1..6 as
+ * the reference numbers faces, 6 treated as 5), GLL face quadrature, assembled
+ * with ADD_VALUES; constrained DOFs dropped.  nface: host, element order
+ * e = ex + nex*(ey + ney*ez).  Output: the nonzero entries of B as (eq, val)
+ * pairs in ascending eq (host; pass eq = val = NULL to get *nnz only), ready
+ * as the src_eq / src_val of hr_forward. */
+int hr_surface_force(hr_ctx* ctx, const int* nface, long long* eq, double* val, long long* nnz);
+
+/* Regular-domain energy (TOTAL_ENERGY_RD_PETSC_EXPLICIT, Obsolete.F90:108-150,
+ * called every I_PRINT_STEP steps at Solver_PETSc_Explicit.F90:915-916):
+ * 0.5 * (M_RD x3) . x3 -- the reference's AK_RD term is zero because AK_RD is
+ * never assembled (Assemble_PETSc_3D.F90:163).  hr_set_energy_interval(every)
+ * makes later RK4 forwards record it at steps 0, every, 2 every, ... <= nstep
+ * (0: off); hr_get_energy returns the last forward's values, nshot x nrow. */
+int hr_set_energy_interval(hr_ctx* ctx, int every);
+int hr_get_energy(hr_ctx* ctx, double* energy, int* nrow);
+
 /* Gradient post-processing on the regular domain (Program_Main.F90:416-484).
  * Nodal vectors have nnode entries; only regular-domain elements contribute
  * (MTYPE == 1).  wreg: NULL for Weighted_Regularization = 'No', else the host

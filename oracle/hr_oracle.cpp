@@ -793,6 +793,7 @@ struct System {
   CSR AT[3];                       // explicit transposes
   std::vector<i64> tmap[3];        // AT entry -> A entry
   std::vector<double> diag_m;
+  std::vector<double> diag_m_rd;   // DIAG_M_RD: regular-domain elements only
   bool have_t = false;
 };
 
@@ -875,6 +876,9 @@ void assemble(System& S) {
   S.diag_m.assign(S.neq, 0.0);
   for (i64 e = 0; e < nel; ++e) for (auto& p : dm_rd[e]) S.diag_m[p.first] += p.second;
   for (i64 e = 0; e < nel; ++e) for (auto& p : dm_pml[e]) S.diag_m[p.first] += p.second;
+  // DIAG_M_RD (ASSEM_REGULAR_DOMAIN_PETSC_3D, Assemble_PETSc_3D.F90:175)
+  S.diag_m_rd.assign(S.neq, 0.0);
+  for (i64 e = 0; e < nel; ++e) for (auto& p : dm_rd[e]) S.diag_m_rd[p.first] += p.second;
 }
 
 void ensure_transposes(System& S) {
@@ -1070,8 +1074,12 @@ void or_matmult_add(void* s, int which, int transpose, const double* x, const do
 // column n = stored entries of x2 at the start of step n (U_Store_Mapping).
 // The reference loop also advances once more after storing column NSTEP; that
 // state is never read and is not computed here.
-int or_rk4_forward(void* s, const double* Bin, int nstep, double dt, int lcase, double fr, i64 nstore,
-                   const i64* store_eq, double* u_store) {
+// energy (may be null): TOTAL_ENERGY_RD_PETSC_EXPLICIT (Obsolete.F90:108-150) at the
+// start of every step ISTEP with MOD(ISTEP, every) == 0 (Solver_PETSc_Explicit.F90:
+// 915-916): 0.5 * (ENERGY_1 + ENERGY_2), ENERGY_1 = u . AK_RD u = 0 (AK_RD is never
+// filled), ENERGY_2 = (DIAG_M_RD * x3) . x3.
+static int rk4_forward_impl(void* s, const double* Bin, int nstep, double dt, int lcase, double fr, i64 nstore,
+                     const i64* store_eq, double* u_store, int every, double* energy) {
   System* S = (System*)s;
   const i64 n = S->neq;
   std::vector<double> iM(n);
@@ -1092,6 +1100,11 @@ int or_rk4_forward(void* s, const double* Bin, int nstep, double dt, int lcase, 
   for (int istep = 0; istep <= nstep; ++istep) {
     const double T = (double)istep * dt;
     for (i64 r = 0; r < nstore; ++r) u_store[r + nstore * (i64)istep] = x2[store_eq[r]];
+    if (energy && every > 0 && istep % every == 0) {
+      double e2 = 0.0;
+      for (i64 i = 0; i < n; ++i) e2 += (S->diag_m_rd[i] * x3[i]) * x3[i];
+      energy[istep / every] = 0.50 * (0.0 + e2);
+    }
     if (istep == nstep) break;
     const double fac7 = time_function(T, lcase, fr);
     const double fac8 = time_function(T + dt, lcase, fr);
@@ -1133,6 +1146,21 @@ int or_rk4_forward(void* s, const double* Bin, int nstep, double dt, int lcase, 
     axpy(x3, dt / 6.0, help);
   }
   return 0;
+}
+
+int or_rk4_forward(void* s, const double* Bin, int nstep, double dt, int lcase, double fr, i64 nstore,
+                   const i64* store_eq, double* u_store) {
+  return rk4_forward_impl(s, Bin, nstep, dt, lcase, fr, nstore, store_eq, u_store, 0, nullptr);
+}
+
+int or_rk4_forward_energy(void* s, const double* Bin, int nstep, double dt, int lcase, double fr, i64 nstore,
+                          const i64* store_eq, double* u_store, int every, double* energy) {
+  return rk4_forward_impl(s, Bin, nstep, dt, lcase, fr, nstore, store_eq, u_store, every, energy);
+}
+
+void or_get_diag_m_rd(void* s, double* m) {
+  const System* S = (const System*)s;
+  for (i64 i = 0; i < S->neq; ++i) m[i] = S->diag_m_rd[i];
 }
 
 // EXPLICIT_Central_Difference_3 (Obsolete.F90:815-995): the reference's

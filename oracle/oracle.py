@@ -53,6 +53,9 @@ def lib():
         L.or_matmult_add.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.or_rk4_forward.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_int, C.c_double,
                                      C.c_int64, _ip, _dp]
+        L.or_rk4_forward_energy.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_int, C.c_double,
+                                            C.c_int64, _ip, _dp, C.c_int, _dp]
+        L.or_get_diag_m_rd.argtypes = [C.c_void_p, _dp]
         L.or_cd_forward.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_int, C.c_double,
                                     C.c_int64, _ip, _dp]
         L.or_rk4_adjoint.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int64, _ip, _dp,
@@ -259,6 +262,23 @@ class System:
         m = np.zeros(self.neq)
         lib().or_get_diag_m(self.h, m)
         return m
+
+    def diag_m_rd(self):
+        """DIAG_M_RD: the lumped mass of the regular-domain elements only."""
+        m = np.zeros(self.neq)
+        lib().or_get_diag_m_rd(self.h, m)
+        return m
+
+    def forward_energy(self, B, nstep, dt, every, lcase=2, fr=15.0):
+        """The RK4 forward with the regular-domain energy at steps 0, every, ...:
+        returns (U_store, energy)."""
+        store_eq = np.ascontiguousarray(self.mesh.store_eq, dtype=np.int64)
+        ns = len(store_eq)
+        u = np.zeros((nstep + 1) * ns)
+        e = np.zeros(nstep // every + 1)
+        lib().or_rk4_forward_energy(self.h, np.ascontiguousarray(B, dtype=np.float64), int(nstep), float(dt),
+                                    int(lcase), float(fr), ns, store_eq, u, int(every), e)
+        return u.reshape(nstep + 1, ns), e
 
     def gradient_mass(self, wreg=None):
         """Nodal 1/Mg over regular-domain elements (0 where Mg <= 1e-10) and

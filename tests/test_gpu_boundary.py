@@ -1,0 +1,138 @@
+"""The drop-in boundary beyond the time loops (runs on the GPU):
+
+* hr_gradient_accumulate, the Update_g replacement, driven step by step over
+  the oracle's forward and adjoint histories, reproduces the oracle's control
+  pass (Solver_PETSc_Explicit.F90:1474-1510, then g <- -g);
+* hr_set_numbering with the partitioner's node-major numbering of a permuted
+  node order (PIC_ReNumbering.F90:263-285): products, the lumped mass and a
+  forward solve are the lattice-numbered results permuted, bit for bit;
+* hr_surface_force (ASSEM_FORCE_PETSC_3D + ELEMENT_FS_3D) equals the oracle's
+  assembly of the flang-pinned element routine, and drives a forward solve;
+* the regular-domain energy diagnostic against the oracle's.
+Bars: gradients and traces <= 1e-10 relative L2, force <= 1e-14 relative,
+energy <= 1e-10 relative L2.
+"""
+import numpy as np
+import pytest
+
+from hirensi_amd import solver as S
+from hirensi_amd.mesh import BoxMesh, receiver_dofs, surface_grid_nodes
+from oracle import oracle as O
+from tests.helpers import random_model, rel_l2, small_case
+
+pytestmark = pytest.mark.gpu
+
+
+def test_gradient_accumulate_reproduces_the_control_pass():
+    mesh, model, rec_eq, src_eq, src_val = small_case(seed=7)
+    nstep, dt, fr = 120, 1e-4, 60.0
+    print(f"Update_g case neq={mesh.neq} nstep={nstep}", flush=True)
+    s = O.System(mesh, model)
+    B = np.zeros(mesh.neq)
+    np.add.at(B, src_eq, src_val)
+    U = s.forward(B, nstep, dt, lcase=2, fr=fr)
+    pos = {int(e): r for r, e in enumerate(mesh.store_eq)}
+    rows = np.array([pos[int(e)] for e in rec_eq])
+    dmeas = U[:, rows] * 0.8
+    P = s.adjoint(U, nstep, dt, rec_eq, dmeas)
+    glr, gmr = s.control(U, P, nstep, dt)
+    ctx = S.Context(mesh, model)
+    gl, gm = np.zeros(mesh.nnode), np.zeros(mesh.nnode)
+    u, p = np.zeros(mesh.neq), np.zeros(mesh.neq)
+    for n in range(1, nstep):
+        u[mesh.store_eq] = U[n]
+        p[mesh.store_eq] = P[n]
+        ctx.gradient_accumulate(u, p, dt, gl, gm)
+    assert np.abs(glr).max() > 0
+    assert rel_l2(-gl, glr) < 1e-10 and rel_l2(-gm, gmr) < 1e-10
+
+
+def test_caller_numbering_is_the_permuted_lattice_numbering():
+    mesh, model, rec_eq, src_eq, src_val = small_case(nex=7, ney=6, nez=5, seed=8)
+    perm = np.random.default_rng(5).permutation(mesh.nnode)
+    ID2 = mesh.petsc_numbering(perm)
+    new = np.empty(mesh.neq, dtype=np.int64)              # lattice equation -> caller equation
+    new[mesh.ID[mesh.ID >= 0]] = ID2[mesh.ID >= 0]
+    print(f"numbering case neq={mesh.neq}", flush=True)
+    c1 = S.Context(mesh, model)
+    c2 = S.Context(mesh, model)
+    c2.set_numbering(ID2)
+    rng = np.random.default_rng(9)
+    x, y = rng.standard_normal(mesh.neq), rng.standard_normal(mesh.neq)
+    x2, y2 = np.empty_like(x), np.empty_like(y)
+    x2[new], y2[new] = x, y
+    for which in (S.HR_K, S.HR_C, S.HR_G):
+        for tr in (0, 1):
+            z1 = S.ShellMat(c1, which)._apply(tr, x, y, None)
+            z2 = S.ShellMat(c2, which)._apply(tr, x2, y2, None)
+            assert np.array_equal(z2[new], z1), (which, tr)
+    assert np.array_equal(c2.diag_m()[new], c1.diag_m())
+    nstep, dt = 80, 1e-4
+    ftab = S.time_table(nstep, dt, 2, 60.0)
+    t1 = S.explicit_rk4_petsc_pml_3d(c1, nstep, dt, ftab, src_eq, src_val, rec_eq)
+    t2 = S.explicit_rk4_petsc_pml_3d(c2, nstep, dt, ftab, new[src_eq], src_val, new[rec_eq])
+    assert np.abs(t1).max() > 0 and np.array_equal(t1, t2)
+    # a numbering that disagrees with the DOF rules is refused
+    bad = ID2.copy()
+    bad[0, 0] = 0
+    with pytest.raises(S.HrError):
+        c2.set_numbering(bad)
+
+
+def test_surface_force_matches_oracle_assembly():
+    mesh = BoxMesh(8, 7, 6, npml=(2, 1, 1, 2, 2), h=(1.0, 1.25, 0.8))
+    model = random_model(mesh, seed=71)
+    rng = np.random.default_rng(2)
+    nface = rng.integers(0, 7, mesh.nel).astype(np.int32)
+    # the traction sits on the free surface in practice: every top regular-domain element
+    top = np.arange(mesh.nel) >= mesh.nex * mesh.ney * (mesh.nez - 1)
+    nface[top] = 5
+    print(f"surface force case nel={mesh.nel}", flush=True)
+    ctx = S.Context(mesh, model)
+    eq, val = ctx.surface_force(nface)
+    # oracle: ELEMENT_FS_3D assembled with ADD over regular-domain elements
+    orc = O.oracle_elem()
+    xyz, inod, ID = mesh.xyz(), mesh.inod(), mesh.ID
+    Bref = np.zeros(mesh.neq)
+    for e in np.flatnonzero(mesh.mtel == 1):
+        if nface[e] == 0:
+            continue
+        nodes = inod[e]
+        efs = orc.element_fs(xyz[nodes], int(nface[e]))
+        nd1 = np.concatenate([ID[nodes, c] for c in range(3)])
+        ok = nd1 >= 0
+        np.add.at(Bref, nd1[ok], efs[ok])
+    B = np.zeros(mesh.neq)
+    B[eq] = val
+    assert np.all(np.diff(eq) > 0) and np.count_nonzero(Bref) == len(eq)
+    assert np.max(np.abs(B - Bref)) <= 1e-14 * np.abs(Bref).max()
+    rec_eq = receiver_dofs(mesh, surface_grid_nodes(mesh, 2, 2, -1.0, 1.0, -1.25, 1.25))
+    nstep, dt, fr = 100, 1e-4, 60.0
+    tr = S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, S.time_table(nstep, dt, 2, fr), eq, val, rec_eq)
+    U = O.System(mesh, model).forward(Bref, nstep, dt, lcase=2, fr=fr)
+    pos = {int(e): r for r, e in enumerate(mesh.store_eq)}
+    ref = U[:, [pos[int(e)] for e in rec_eq]]
+    assert np.abs(ref).max() > 0 and rel_l2(tr, ref) < 1e-10
+
+
+def test_energy_diagnostic_matches_oracle():
+    mesh, model, rec_eq, src_eq, src_val = small_case(seed=9)
+    nstep, dt, fr, every = 200, 1e-4, 60.0, 25
+    print(f"energy case neq={mesh.neq} every={every}", flush=True)
+    ctx = S.Context(mesh, model)
+    ctx.set_energy_interval(every)
+    S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, S.time_table(nstep, dt, 2, fr), src_eq, src_val, rec_eq)
+    E = ctx.energy()[0]
+    B = np.zeros(mesh.neq)
+    np.add.at(B, src_eq, src_val)
+    _, Eref = O.System(mesh, model).forward_energy(B, nstep, dt, every, lcase=2, fr=fr)
+    assert E.shape == Eref.shape and Eref[1:].min() > 0
+    assert rel_l2(E, Eref) < 1e-10
+    # batched shots record one row per shot
+    ftab = np.stack([S.time_table(nstep, dt, 2, fr)] * 2)
+    S.explicit_rk4_petsc_pml_3d_shots(ctx, nstep, dt, ftab, np.array([0, 1], np.int32),
+                                      np.concatenate([src_eq, src_eq]), np.concatenate([src_val, 2 * src_val]),
+                                      rec_eq)
+    E2 = ctx.energy()
+    assert E2.shape == (2, len(Eref))
+    assert np.array_equal(E2[0], E) and rel_l2(E2[1], 4.0 * E) < 1e-12

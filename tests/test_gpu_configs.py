@@ -183,23 +183,30 @@ class _DeviceSolve:
 def test_config2_device_pointers_against_oracle():
     mesh, true_cfg, start_cfg, rec_eq, src_eq = _config2_setup()
     # 250 steps of a 60 Hz Ricker: the pulse has passed and its reflections from the
-    # anomaly reach the receivers (15 Hz would still be rising at 0.025 s)
+    # anomaly reach the receivers (15 Hz would still be rising at 0.025 s).  The data
+    # are half the anomaly model's traces, so that the residual u - d is of the order
+    # of u: with the anomaly's own traces the residual is a small difference of nearly
+    # equal traces, and FP64 summation-order differences in u (~1e-14) are amplified
+    # by |u| / |u - d| in the adjoint source and the gradient.
     nstep, dt, fr = 250, true_cfg["dt"], 60.0
     print(f"config2 device-pointer case neq={mesh.neq} nstep={nstep} nrec={len(rec_eq)}", flush=True)
     ds = _DeviceSolve(mesh, rec_eq, src_eq, nstep, dt, fr)
     ds.material(true_cfg["model"])
-    dobs = ds.forward()
+    dtrue = ds.forward()
+    dobs = 0.5 * dtrue
     ds.material(start_cfg["model"])
     tr = ds.forward(history=S.HR_HISTORY_STORE).cpu().numpy()
     gl, gm, mis = ds.gradient(dobs)
     dobs_h = dobs.cpu().numpy()
     ds.ctx.close()
-    print("config2 GPU done; oracle run", flush=True)
+    print(f"config2 GPU done (|u - d_anomaly| / |u| = {rel_l2(tr, dtrue.cpu().numpy()):.2e}); oracle run",
+          flush=True)
     ref = _oracle_run(mesh, start_cfg["model"], src_eq, np.ones(1), rec_eq, nstep, dt, fr, dmeas=dobs_h)
+    errs = (rel_l2(tr, ref["traces"]), rel_l2(gl, ref["gl"]), rel_l2(gm, ref["gm"]),
+            abs(mis - ref["misfit"]) / ref["misfit"])
+    print("config2 errors: traces %.2e g_lambda %.2e g_mu %.2e misfit %.2e" % errs, flush=True)
     assert np.abs(ref["gm"]).max() > 0 and ref["misfit"] > 0
-    assert rel_l2(tr, ref["traces"]) < 1e-10
-    assert rel_l2(gl, ref["gl"]) < 1e-10 and rel_l2(gm, ref["gm"]) < 1e-10
-    assert abs(mis - ref["misfit"]) <= 1e-12 * ref["misfit"]
+    assert errs[0] < 1e-10 and errs[1] < 1e-10 and errs[2] < 1e-10 and errs[3] <= 1e-12
 
 
 @pytest.mark.timeout(900)

@@ -194,3 +194,71 @@ def test_energy_conserved_without_pml_damping():
     assert abs(energy(x, v) - e0) <= 1e-6 * e0
     # with beta0 = 0 the G blocks (products of betas) vanish
     assert G.nnz == 0 or np.abs(G.data).max() == 0.0
+
+
+def test_energy_diagnostic_matches_textbook():
+    """or_rk4_forward_energy: 0.5 (M_RD x3) . x3 every `every` steps, with M_RD the
+    regular-domain elements' lumped mass (element_m flag 201 assembled on its own)
+    and x3 from the textbook RK4 of the assembled system."""
+    mesh, model, rec_eq, src_eq, src_val = small_case(nex=6, ney=6, nez=5, npml=2, seed=2)
+    s = O.System(mesh, model)
+    B = np.zeros(mesh.neq)
+    B[src_eq] = src_val
+    nstep, dt, fr, every = 90, 1e-4, 60.0, 15
+    U, E = s.forward_energy(B, nstep, dt, every, lcase=2, fr=fr)
+    assert np.array_equal(U, s.forward(B, nstep, dt, lcase=2, fr=fr))
+    # M_RD from the element routine
+    orc = O.oracle_elem()
+    xyz, inod, ID = mesh.xyz(), mesh.inod(), mesh.ID
+    mrd = np.zeros(mesh.neq)
+    for e in np.flatnonzero(mesh.mtel == 1):
+        nodes = inod[e]
+        nd1 = np.concatenate([ID[nodes, c] for c in range(3)])
+        d = np.diag(orc.element_m(xyz[nodes], model.rho[e], mesh.pml_param(), 201))
+        ok = nd1 >= 0
+        np.add.at(mrd, nd1[ok], d[ok])
+    assert np.allclose(s.diag_m_rd(), mrd, rtol=1e-14, atol=0)
+    # x3 of the textbook RK4
+    K, C, G = (s.csr(w) for w in range(3))
+    iM = 1.0 / s.diag_m()
+    Bt = B * iM
+    n = mesh.neq
+    x1, x2, x3 = np.zeros(n), np.zeros(n), np.zeros(n)
+
+    def rhs(a1, a2, a3, f):
+        return a2, a3, -iM * (C @ a3 + K @ a2 + G @ a1) + f * Bt
+
+    ref = []
+    for k in range(nstep + 1):
+        if k % every == 0:
+            ref.append(0.5 * np.dot(mrd * x3, x3))
+        if k == nstep:
+            break
+        T = k * dt
+        f0, fh, f1 = (O.time_function(t, 2, fr) for t in (T, T + 0.5 * dt, T + dt))
+        a = rhs(x1, x2, x3, f0)
+        b = rhs(x1 + 0.5 * dt * a[0], x2 + 0.5 * dt * a[1], x3 + 0.5 * dt * a[2], fh)
+        c = rhs(x1 + 0.5 * dt * b[0], x2 + 0.5 * dt * b[1], x3 + 0.5 * dt * b[2], fh)
+        d = rhs(x1 + dt * c[0], x2 + dt * c[1], x3 + dt * c[2], f1)
+        x1 = x1 + dt / 6 * (a[0] + 2 * b[0] + 2 * c[0] + d[0])
+        x2 = x2 + dt / 6 * (a[1] + 2 * b[1] + 2 * c[1] + d[1])
+        x3 = x3 + dt / 6 * (a[2] + 2 * b[2] + 2 * c[2] + d[2])
+    ref = np.array(ref)
+    assert E[0] == 0.0 and ref[1:].min() > 0
+    assert rel_l2(E, ref) < 1e-12
+
+
+def test_petsc_numbering_of_a_node_order():
+    mesh = BoxMesh(5, 4, 3, npml=(1, 1, 1, 1, 1))
+    assert np.array_equal(mesh.petsc_numbering(np.arange(mesh.nnode)), mesh.ID)
+    perm = np.random.default_rng(3).permutation(mesh.nnode)
+    ID = mesh.petsc_numbering(perm)
+    assert np.array_equal(ID >= 0, mesh.ID >= 0)
+    eqs = ID[ID >= 0]
+    assert np.array_equal(np.sort(eqs), np.arange(mesh.neq))
+    # node-major in the given order, DOFs interleaved
+    first = ID[perm].max(axis=1)
+    act = first >= 0
+    assert np.all(np.diff(first[act]) > 0)
+    with pytest.raises(ValueError):
+        mesh.petsc_numbering(np.zeros(mesh.nnode, dtype=np.int64))
