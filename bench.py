@@ -35,12 +35,20 @@ def parse():
     p.add_argument("--version", type=int, default=0)
     p.add_argument("--serial-classes", action="store_true", help="v3: run the tile classes one after another")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--ckpt-budget", type=float, default=8.0e9,
+                   help="HBM budget (bytes) of the checkpointed leg (binomial revolve history); 0 skips the leg")
     p.add_argument("--cpu-steps", type=int, default=40, help="oracle RK4 steps timed (each direction)")
     p.add_argument("--copy-calib", action="store_true", help="also run a plain 1 GiB device copy (PMC scale)")
     p.add_argument("--shots", type=int, default=0,
                    help="instead of the default line: batched multi-shot gradient on the 48x48x24 survey "
                         "(use with --nstep; shots/s against single-shot solves)")
     p.add_argument("--seq-shots", type=int, default=2, help="single-shot gradients timed for the comparison")
+    p.add_argument("--fwi", choices=["config4", "small"], default=None,
+                   help="instead of the default line: L-BFGS FWI on config 4 (96x96x48, 9 shots) or its 48x48x24 "
+                        "reduction, plain sequential and/or batched + checkpointed (use with --nstep, --fwi-iters)")
+    p.add_argument("--fwi-iters", type=int, default=20)
+    p.add_argument("--fwi-variant", choices=["both", "batched", "plain"], default="both")
+    p.add_argument("--fwi-budget-s", type=float, default=900.0, help="stop a variant after this many seconds")
     p.add_argument("--fd-check", action="store_true",
                    help="instead of timing: check the lambda/mu gradient against central finite differences "
                         "along 5 seeded smooth directions on this mesh (use with --nstep)")
@@ -77,6 +85,41 @@ def measured_traffic():
         return json.load(f)
 
 
+def _log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_config1(threads):
+    """The oracle on config 1 in full (16x16x8, PML 4, 2000 RK4 steps, forward
+    only, the reference's CPU-runnable case), assembly excluded."""
+    import numpy as np
+    from oracle import oracle as O
+    from hirensi_amd.mesh import config1
+    c = config1()
+    mesh = c["mesh"]
+    O.lib().or_set_threads(threads)
+    s = O.System(mesh, c["model"])
+    B = np.zeros(mesh.neq)
+    B[mesh.dof(c["src_node"], 2)] = 1.0
+    t0 = time.perf_counter()
+    s.forward(B, c["nstep"], c["dt"], lcase=2, fr=c["fr"])
+    wall = time.perf_counter() - t0
+    s.close()
+    return {"value": mesh.neq * c["nstep"] / wall, "unit": "DOF-steps/s", "wall_s": wall, "cores": threads,
+            "sample": f"config 1 in full: 16x16x8 el, PML 4, {c['nstep']} RK4 forward steps, neq {mesh.neq}"}
+
+
 def cpu_baseline(cfg, nsteps):
     """The oracle (CSR + PETSc-sequence RK4, OpenMP) on the same mesh and model,
     timed over `nsteps` forward and `nsteps` adjoint steps; assembly excluded."""
@@ -99,8 +142,10 @@ def cpu_baseline(cfg, nsteps):
     s.close()
     wall = (t1 - t0) + (t3 - t2)
     return {"value": mesh.neq * 2 * nsteps / wall, "unit": "DOF-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": _cpu_model(),
             "sample": f"oracle on the same 64x64x32 mesh and model, {nsteps} forward + {nsteps} adjoint "
-                      f"RK4 steps, assembly excluded ({wall:.1f} s)"}
+                      f"RK4 steps, assembly excluded ({wall:.1f} s)",
+            "config1_full": cpu_config1(threads)}
 
 
 def fd_check(a):
@@ -235,8 +280,81 @@ def shots_bench(a):
         "sequential_shots_per_s": 1.0 / t_one, "speedup_vs_sequential": t_one * nshot / t_batch}), flush=True)
 
 
+def fwi_bench(a):
+    """L-BFGS FWI (m = 5) of the config-4 survey: the plain form (one shot per
+    solve, stored history) against batched shots with the binomially
+    checkpointed history, same algorithm and data; misfit and mu model error per
+    iteration, wall time, and the batched run's time to reach the plain run's
+    final misfit.  Prints one JSON line."""
+    import numpy as np
+    from hirensi_amd import solver as S
+    from hirensi_amd import fwi as F
+    from hirensi_amd.mesh import config4
+    cfg = config4() if a.fwi == "config4" else config4(nx=48, ny=48, nz=24, npml=6, nrec_side=15)
+    mesh = cfg["mesh"]
+    nstep = a.nstep if a.nstep != 4000 else cfg["nstep"]
+    sv = F.survey_of(cfg, nstep)
+    params = F.InversionParams(m_lbfgs=5, bound_lambda_lower=40.0, bound_lambda_upper=1200.0, bound_mu_lower=20.0,
+                               bound_mu_upper=600.0, fac_max_update_lambda=10.0, fac_max_update_mu=10.0)
+    ctx = S.Context(mesh)
+    ctx.set_material(cfg["true"])
+    _log(f"fwi {a.fwi}: neq {mesh.neq}, {len(cfg['src_nodes'])} shots, {nstep} steps; observing the true model")
+    obs = F.HipProblem(ctx, sv, cfg["true"].rho).observe(cfg["true"].lam, cfg["true"].mu)
+    sv.dis_meas = obs
+    _, restriction = S.gradient_mass(ctx)
+    start, true = cfg["start"], cfg["true"]
+    runs = {}
+    variants = ["plain", "batched"] if a.fwi_variant == "both" else [a.fwi_variant]
+    for v in variants:
+        if v == "plain":
+            prob = F.SequentialHipProblem(ctx, sv, start.rho, params)
+        else:
+            ctx.set_history_budget(0.0)
+            prob = F.HipProblem(ctx, sv, start.rho, params, history=S.HR_HISTORY_REVOLVE)
+        t0 = time.perf_counter()
+        inv = F.Inversion(prob, start.lam, start.mu, restriction, params)
+        recs = [{"iter": 0, "misfit": inv.misfit, "mu_error": F.model_error(mesh, inv.mu, true.mu),
+                 "wall_s": time.perf_counter() - t0}]
+        _log(f"fwi {v}: iteration 0 misfit {inv.misfit:.6e} mu error {recs[0]['mu_error']:.4f}")
+        stop = ""
+        for k in range(a.fwi_iters):
+            try:
+                r = inv.iterate()
+            except F.ArmijoFailure as e:
+                stop = str(e)
+                break
+            recs.append({"iter": r["iter"], "misfit": r["misfit"], "mu_error": F.model_error(mesh, inv.mu, true.mu),
+                         "armijo_cuts": r["armijo_counter"], "wall_s": time.perf_counter() - t0})
+            _log(f"fwi {v}: iteration {r['iter']} misfit {r['misfit']:.6e} mu error {recs[-1]['mu_error']:.4f} "
+                 f"({recs[-1]['wall_s']:.1f} s)")
+            if recs[-1]["wall_s"] > a.fwi_budget_s:
+                stop = f"time budget of {a.fwi_budget_s} s reached"
+                break
+        runs[v] = {"records": recs, "forward_solves": prob.nforward, "stopped": stop,
+                   "recompute_ratio": ctx.recompute_ratio()}
+    out = {"metric": f"FWI time-to-misfit ({a.fwi}: {mesh.nex}x{mesh.ney}x{mesh.nez} el, "
+                     f"{len(cfg['src_nodes'])} shots, {nstep} steps, L-BFGS m=5)",
+           "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+           "data": "synthetic (4-layer model with two +-25 % ellipsoids; start: linear Vs 150-450 m/s)",
+           "config": {"workload": a.fwi, "neq": mesh.neq, "nstep": nstep, "nshot": len(cfg["src_nodes"]),
+                      "nrec": len(cfg["rec_nodes"])}, "runs": runs}
+    if "plain" in runs and "batched" in runs:
+        target = runs["plain"]["records"][-1]["misfit"]
+        hit = [r for r in runs["batched"]["records"] if r["misfit"] <= target]
+        tp = runs["plain"]["records"][-1]["wall_s"]
+        if hit:
+            out["value"] = hit[0]["wall_s"]
+            out["plain_time_s"] = tp
+            out["speedup"] = tp / hit[0]["wall_s"]
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
+    if a.fwi:
+        fwi_bench(a)
+        return
     if a.shots:
         shots_bench(a)
         return
@@ -253,6 +371,7 @@ def main():
     from hirensi_amd.mesh import config2, receiver_dofs
 
     dev = torch.device("cuda:0")
+    _log("building the 64x64x32 mesh and models")
     true_cfg = config2(with_anomaly=True)
     start_cfg = config2(with_anomaly=False)
     mesh = true_cfg["mesh"]
@@ -283,22 +402,32 @@ def main():
     sm = start_cfg["model"]
     ctx.set_material_ptrs(T(sm.lam), T(sm.mu), T(sm.rho))
     torch.cuda.synchronize()
+    _log("observed data done")
+    tuned = None
+    if not a.tz:
+        # the stage-kernel z-chunk length for this mesh, from a few timed RK4 steps per candidate
+        tz, ms = ctx.autotune(nshot=1, steps=6)
+        tuned = {"tz": tz, "ms_per_rk4_step": ms}
+        _log(f"autotune: z-chunks of {tz} element layers ({ms:.3f} ms per RK4 step)")
 
     def gradient():
         S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, d_ftab, d_src, d_val, d_rec, history=history,
                                     segment=a.segment, traces=d_tr)
         S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, d_rec, d_obs, d_gl, d_gm, misfit_out=d_mis)
 
-    for _ in range(a.warmup):
+    for w in range(a.warmup):
         gradient()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        _log(f"warmup gradient {w + 1}/{a.warmup} done")
     L = ctx.L
     L.hr_set_profiling(ctx.h, 2)
     L.hr_set_profiling(ctx.h, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for k in range(a.steps):
         gradient()
+        if k + 1 == a.steps or (k + 1) % 5 == 0:
+            _log(f"timed gradient {k + 1}/{a.steps} queued")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     n_l = (np.zeros(5, dtype=np.int64))
@@ -337,7 +466,7 @@ def main():
         "config": {"workload": "fwd+adj+grad, 64x64x32 el, PML 6, 4000 RK4 steps, 400 receivers",
                    "neq": neq, "nnode": mesh.nnode, "nstep": nstep, "history": a.history,
                    "recompute_ratio": ctx.recompute_ratio(), "misfit": misfit, "g_mu_norm": gnorm,
-                   "gradient_wall_s": wall, "parallelism": "single GPU"},
+                   "gradient_wall_s": wall, "parallelism": "single GPU", "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": tr["traffic_bytes_per_launch"] if tr else None,
@@ -350,6 +479,28 @@ def main():
                      "fused_achieved": fused_bytes / (kern_ms * 1e-3) / 1e9,
                      "stage_kernel_share_of_wall": kern_ms * 1e-3 / (wall * a.steps)},
     }
+    if a.ckpt_budget > 0:
+        # the same gradient with the forward history checkpointed in HBM within a budget
+        # (binomial revolve schedule); one warm-up (allocates the snapshot pool), one timed
+        ctx.set_history_budget(a.ckpt_budget)
+        L_, ns_, nb_, ratio_ = ctx.history_plan(nstep, S.HR_HISTORY_REVOLVE)
+
+        def gradient_ckpt():
+            S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, d_ftab, d_src, d_val, d_rec, history=S.HR_HISTORY_REVOLVE,
+                                        traces=d_tr)
+            S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, d_rec, d_obs, d_gl, d_gm, misfit_out=d_mis)
+
+        gradient_ckpt()
+        torch.cuda.synchronize()
+        t0c = time.perf_counter()
+        gradient_ckpt()
+        torch.cuda.synchronize()
+        wall_c = time.perf_counter() - t0c
+        _log(f"checkpointed gradient done ({wall_c:.2f} s, recompute ratio {ctx.recompute_ratio():.3f})")
+        out["checkpointed"] = {"history": "revolve", "budget_bytes": a.ckpt_budget, "history_bytes": nb_,
+                               "segment": L_, "snapshots": ns_, "recompute_ratio": ctx.recompute_ratio(),
+                               "gradient_wall_s": wall_c, "value": dof_steps / wall_c, "unit": "DOF-steps/s",
+                               "store_history_bytes": 8.0 * ctx.nstore * (nstep + 1)}
     if a.copy_calib:
         x = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         y = torch.empty_like(x)
@@ -358,6 +509,7 @@ def main():
         torch.cuda.synchronize()
     ctx.close()
     if not a.no_cpu_baseline:
+        _log("CPU baseline (oracle)")
         cfg = {"mesh": mesh, "start": sm, "src_eq": src_eq, "rec_eq": rec_eq, "dt": dt, "fr": fr}
         out["cpu_baseline"] = cpu_baseline(cfg, a.cpu_steps)
     print(json.dumps(out), flush=True)

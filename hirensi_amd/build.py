@@ -42,6 +42,29 @@ def build_hip(extra=None):
     return LIB
 
 
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+FDRIVER = os.path.join(HERE, "fortran", "hr_driver")
+
+
+def build_fortran():
+    """The Fortran driver of the C ABI (hirensi_amd/fortran/hr_driver), linked
+    against libhirensi.so next to it (rpath $ORIGIN/..), where ROCm flang exists."""
+    if not os.path.exists(FLANG):
+        print("ROCm flang absent: Fortran driver not built", flush=True)
+        return False
+    import tempfile
+    fdir = os.path.join(HERE, "fortran")
+    with tempfile.TemporaryDirectory() as tmp:
+        _run([FLANG, "-O2", "-c", os.path.join(fdir, "hirensi_mod.F90"), "-o", "m.o"], cwd=tmp)
+        _run([FLANG, "-O2", "-c", os.path.join(fdir, "hr_driver.F90"), "-o", "d.o"], cwd=tmp)
+        rt = os.path.join(os.path.dirname(os.path.dirname(FLANG)), "lib")
+        tmpexe = FDRIVER + ".tmp"
+        _run([FLANG, "d.o", "m.o", "-o", tmpexe, f"-L{HERE}", "-lhirensi", "-Wl,-rpath,$ORIGIN/..",
+              f"-Wl,-rpath,{rt}"], cwd=tmp)
+        os.replace(tmpexe, FDRIVER)
+    return True
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
@@ -58,6 +81,7 @@ def build_reference():
 
 def build_all():
     build_hip()
+    build_fortran()
     build_oracle()
     build_reference()
 
@@ -65,4 +89,5 @@ def build_all():
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "hip": build_hip, "oracle": build_oracle, "ref": build_reference}[w]()
+        {"all": build_all, "hip": build_hip, "oracle": build_oracle, "ref": build_reference,
+         "fortran": build_fortran}[w]()

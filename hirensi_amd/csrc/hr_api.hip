@@ -1678,6 +1678,50 @@ int hr_get_history_plan(hr_ctx* ctx, int nstep, int nshot, int history, int segm
   return rc;
 }
 
+int hr_autotune(hr_ctx* ctx, int nshot, int steps, int* tz_best, double* ms_per_step) {
+  if (int rc = check_ready(ctx)) return rc;
+  if (nshot < 1 || steps < 1) return fail(ctx, 1, "autotune needs nshot >= 1 and steps >= 1");
+  if (int rc = ensure_shots(ctx, nshot)) return rc;
+  ctx->deferred.clear();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  const int tz0 = ctx->g.tz;
+  const double sgn[3] = {1.0, 1.0, 1.0};
+  double best = -1.0;
+  int bt = tz0;
+  int status = 0;
+  // z-chunk candidates: element layers per chunk of the stage-kernel tiles
+  for (int tz : {3, 4, 5, 6, 7, 8, 10, 12}) {
+    if (tz > ctx->g.nez) continue;
+    ctx->g.tz = tz;
+    if (setup_tiles(ctx) != 0) continue;      // too many chunks for this mesh
+    // timed RK4 steps on the zero state without sources: the same launches and traffic
+    if ((status = zero_state(ctx))) break;
+    if ((status = rk4_step(ctx, 1.0e-4, +1, 0, sgn, nullptr))) break;
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) { status = fail(ctx, 3, "hipEventRecord failed"); break; }
+    for (int k = 0; k < steps && !status; ++k) status = rk4_step(ctx, 1.0e-4, +1, 0, sgn, nullptr);
+    if (status) break;
+    float ms = 0.0f;
+    if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      status = fail(ctx, 3, "autotune timing failed");
+      break;
+    }
+    const double per = (double)ms / steps;
+    if (best < 0.0 || per < best) { best = per; bt = tz; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->g.tz = status ? tz0 : bt;
+  if (int rc = setup_tiles(ctx)) return rc;
+  if (int rc = zero_state(ctx)) return rc;
+  if (status) return status;
+  if (tz_best) *tz_best = bt;
+  if (ms_per_step) *ms_per_step = best;
+  return 0;
+}
+
 int hr_set_energy_interval(hr_ctx* ctx, int every) {
   if (!ctx) return 1;
   if (every < 0) return fail(ctx, 1, "energy interval must be >= 0");

@@ -141,6 +141,138 @@ __global__ void gradient_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int b
   }
 }
 
+// The same per-step gradient, tiled: a block owns GTX x GTY box nodes in x-y
+// and marches up a chunk of GZC planes.  Each value of u and p is read from
+// memory once per block (plus a halo of two nodes in x and y and two planes in z):
+// the thread of a column keeps the planes k-2..k+2 of its own u and p in
+// registers (z stencils), the plane k of the block's window sits in LDS (x and y
+// stencils).  Sums and their order are those of gradient_kernel, so the results
+// are identical.
+constexpr int GTX = 32, GTY = 8, GH = 2, GZC = 4;
+constexpr int GWX = GTX + 2 * GH, GWY = GTY + 2 * GH;
+constexpr int GNT = GTX * GTY;
+// halo positions read by the stencils: rows 0, 1 and GWY-2, GWY-1 over the owned
+// columns (y stencils), columns 0, 1 and GWX-2, GWX-1 over the owned rows (x stencils)
+constexpr int GHALO = 2 * 2 * GTX + 2 * 2 * GTY;
+
+__global__ void __launch_bounds__(GNT) gradient_tile_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int bny,
+                                                            int bnz, const double* __restrict__ uh,
+                                                            const double* __restrict__ p2, double dt,
+                                                            double* __restrict__ glam, double* __restrict__ gmu) {
+  __shared__ double sU[3][GWY][GWX], sP[3][GWY][GWX];
+  const int tid = threadIdx.x;
+  const int tx = tid % GTX, ty = tid / GTX;
+  const int ti0 = blockIdx.x * GTX, tj0 = blockIdx.y * GTY;
+  const int kc0 = blockIdx.z * GZC, kc1 = min(kc0 + GZC, bnz);
+  const int bi = ti0 + tx, bj = tj0 + ty;
+  const bool own = bi < bnx && bj < bny;
+  const int nbox = bnx * bny * bnz;              // 32-bit: 9 NN < 2^31 (checked by the host)
+  // u and p of box node (i, j, k) of this block's window (zero outside the box)
+  auto fetch = [&](int i, int j, int k, double* u, double* p) {
+    if (i >= 0 && j >= 0 && k >= 0 && i < bnx && j < bny && k < bnz) {
+      const int r = i + bnx * (j + bny * k);
+      const int n = (bi0 + i) + g.NX * ((bj0 + j) + g.NY * (bk0 + k));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        u[c] = uh[c * nbox + r];
+        p[c] = p2[3 * n + c];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { u[c] = 0.0; p[c] = 0.0; }
+    }
+  };
+  // own column: planes k-2 .. k+2 (index 0..4)
+  double qu[5][3], qp[5][3];
+#pragma unroll
+  for (int d = 0; d < 4; ++d) fetch(bi, bj, kc0 - 2 + d, qu[d], qp[d]);
+  const int wx = GH + tx, wy = GH + ty;
+  for (int k = kc0; k < kc1; ++k) {
+    fetch(bi, bj, k + 2, qu[4], qp[4]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { sU[c][wy][wx] = qu[2][c]; sP[c][wy][wx] = qp[2][c]; }
+    for (int h = tid; h < GHALO; h += GNT) {
+      int hx, hy;
+      if (h < 4 * GTX) { hy = h / GTX; hx = GH + (h - hy * GTX); if (hy >= 2) hy += GTY; }
+      else { const int q = h - 4 * GTX; hy = GH + (q >> 2); hx = q & 3; if (hx >= 2) hx += GTX; }
+      double u[3], p[3];
+      fetch(ti0 + hx - GH, tj0 + hy - GH, k, u, p);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { sU[c][hy][hx] = u[c]; sP[c][hy][hx] = p[c]; }
+    }
+    __syncthreads();
+    if (own) {
+      const int pos[3] = {bi, bj, k};
+      const int bn[3] = {bnx, bny, bnz};
+      // per axis two element sides: s = 0 the element below (local 2; local 1 for an
+      // odd node), s = 1 the element above (local 0), within the box
+      bool has[3][2];
+      int lo[3][2];
+      double du[3][2][3], dp[3][2][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int q = pos[a];
+        const bool odd = q & 1;
+        lo[a][0] = odd ? 1 : 2;
+        lo[a][1] = 0;
+        has[a][0] = odd ? true : q >= 2;
+        has[a][1] = !odd && q + 2 <= bn[a] - 1;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int l = lo[a][s];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            double su = 0.0, sp = 0.0;
+            if (has[a][s]) {
+#pragma unroll
+              for (int m = 0; m < 3; ++m) {
+                const int off = m - l;
+                double uv, pv;
+                if (a == 0) { uv = sU[c][wy][wx + off]; pv = sP[c][wy][wx + off]; }
+                else if (a == 1) { uv = sU[c][wy + off][wx]; pv = sP[c][wy + off][wx]; }
+                else { uv = qu[2 + off][c]; pv = qp[2 + off][c]; }
+                su += DM(l, m) * uv;
+                sp += DM(l, m) * pv;
+              }
+            }
+            du[a][s][c] = g.dxi[a] * su;
+            dp[a][s][c] = g.dxi[a] * sp;
+          }
+        }
+      }
+      double sl = 0.0, sm = 0.0;
+#pragma unroll
+      for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+        for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+          for (int sz = 0; sz < 2; ++sz) {
+            if (!(has[0][sx] && has[1][sy] && has[2][sz])) continue;
+            const double u1x = du[0][sx][0], u2x = du[0][sx][1], u3x = du[0][sx][2];
+            const double u1y = du[1][sy][0], u2y = du[1][sy][1], u3y = du[1][sy][2];
+            const double u1z = du[2][sz][0], u2z = du[2][sz][1], u3z = du[2][sz][2];
+            const double p1x = dp[0][sx][0], p2x = dp[0][sx][1], p3x = dp[0][sx][2];
+            const double p1y = dp[1][sy][0], p2y = dp[1][sy][1], p3y = dp[1][sy][2];
+            const double p1z = dp[2][sz][0], p2z = dp[2][sz][1], p3z = dp[2][sz][2];
+            const double sL = (u1x + u2y + u3z) * (p1x + p2y + p3z);
+            const double sM = 2.0 * (u1x * p1x + u2y * p2y + u3z * p3z) + (p1y + p2x) * (u1y + u2x) +
+                              (p1z + p3x) * (u1z + u3x) + (p3y + p2z) * (u3y + u2z);
+            const double W = g.wq[lo[0][sx] + 3 * lo[1][sy] + 9 * lo[2][sz]];
+            sl += sL * W;
+            sm += sM * W;
+          }
+      const int r = bi + bnx * (bj + bny * k);
+      glam[r] += sl * dt;
+      gmu[r] += sm * dt;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { qu[d][c] = qu[d + 1][c]; qp[d][c] = qp[d + 1][c]; }
+  }
+}
+
 // dst[n][q] = src[n][perm[q]]: receiver columns in source-entry order
 __global__ void permute_cols_kernel(long long nrows, int ncols, const long long* __restrict__ perm,
                                     const double* __restrict__ src, double* __restrict__ dst) {
@@ -285,6 +417,15 @@ hipError_t launch_store_box(const Geo& g, const int* b0, const int* bn, const do
 }
 hipError_t launch_gradient(const Geo& g, const int* b0, const int* bn, const double* uh, const double* p2, double dt,
                            double* glam, double* gmu, hipStream_t s) {
+  const dim3 grid((bn[0] + GTX - 1) / GTX, (bn[1] + GTY - 1) / GTY, (bn[2] + GZC - 1) / GZC);
+  hipLaunchKernelGGL(gradient_tile_kernel, grid, dim3(GNT), 0, s, g, b0[0], b0[1], b0[2], bn[0], bn[1], bn[2], uh, p2,
+                     dt, glam, gmu);
+  return hipGetLastError();
+}
+
+// the node-per-thread form (kept as the reference for the tiled kernel's tests)
+hipError_t launch_gradient_pointwise(const Geo& g, const int* b0, const int* bn, const double* uh, const double* p2,
+                                     double dt, double* glam, double* gmu, hipStream_t s) {
   const long long tot = (long long)bn[0] * bn[1] * bn[2];
   hipLaunchKernelGGL(gradient_kernel, dim3(grid_for(tot)), dim3(256), 0, s, g, b0[0], b0[1], b0[2], bn[0], bn[1],
                      bn[2], uh, p2, dt, glam, gmu);

@@ -16,6 +16,7 @@ module hirensi
   integer(c_int), parameter :: HR_K = 0, HR_C = 1, HR_G = 2
   integer(c_int), parameter :: HR_DEVICE_PTRS = 1
   integer(c_int), parameter :: HR_HISTORY_NONE = 0, HR_HISTORY_STORE = 1, HR_HISTORY_CHECKPOINT = 2
+  integer(c_int), parameter :: HR_HISTORY_REVOLVE = 3
   integer(c_int), parameter :: HR_REG_TV = 0, HR_REG_TN = 1
 
   type, bind(C) :: hr_mesh_desc
@@ -100,6 +101,65 @@ module hirensi
       real(c_double), intent(in) :: x(*), y(*)
       real(c_double), intent(inout) :: z(*)
     end function hr_apply
+
+    ! MatMultAdd(A, x, k, k) passes one vector as both y and z: bind the same
+    ! symbol with c_loc arguments (k declared TARGET), which is conforming
+    ! Fortran where two array dummies sharing an actual argument are not.
+    integer(c_int) function hr_apply_ptr(ctx, which, transpose, x, y, z) bind(C, name="hr_apply")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which, transpose
+      type(c_ptr), value :: x, y, z
+    end function hr_apply_ptr
+
+    integer(c_int) function hr_gradient_accumulate(ctx, u, p, scale, g_lambda, g_mu) &
+        bind(C, name="hr_gradient_accumulate")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: u(*), p(*)
+      real(c_double), value :: scale
+      real(c_double), intent(inout) :: g_lambda(*), g_mu(*)
+    end function hr_gradient_accumulate
+
+    integer(c_int) function hr_set_numbering(ctx, id) bind(C, name="hr_set_numbering")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), intent(in) :: id(9, *)
+    end function hr_set_numbering
+
+    integer(c_int) function hr_surface_force(ctx, nface, eq, val, nnz) bind(C, name="hr_surface_force")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: nface(*)
+      type(c_ptr), value :: eq, val        ! c_loc of the outputs, or c_null_ptr to size
+      integer(c_long_long), intent(out) :: nnz
+    end function hr_surface_force
+
+    integer(c_int) function hr_set_energy_interval(ctx, every) bind(C, name="hr_set_energy_interval")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: every
+    end function hr_set_energy_interval
+
+    integer(c_int) function hr_get_energy(ctx, energy, nrow) bind(C, name="hr_get_energy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, energy
+      integer(c_int), intent(out) :: nrow
+    end function hr_get_energy
+
+    integer(c_int) function hr_set_history_budget(ctx, bytes) bind(C, name="hr_set_history_budget")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: bytes
+    end function hr_set_history_budget
+
+    integer(c_int) function hr_autotune(ctx, nshot, steps, tz_best, ms_per_step) bind(C, name="hr_autotune")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nshot, steps
+      integer(c_int), intent(out) :: tz_best
+      real(c_double), intent(out) :: ms_per_step
+    end function hr_autotune
 
     real(c_double) function hr_time_function(t, lcase, fr) bind(C, name="hr_time_function")
       import :: c_double, c_int

@@ -265,3 +265,68 @@ class HipProblem:
         c_l, c_m, _, _ = S.regularize_gradient(self.ctx, it, g_l, g_m, lam, mu, self.P.regularization,
                                                self.P.regularization_factor, self.wreg)
         return g_l, g_m, c_l, c_m
+
+
+class SequentialHipProblem(HipProblem):
+    """The plain form of the same problem: one shot per solve, as the reference
+    runs them (Solve_Forward.F90:117, one source).  The forward history is kept
+    whole when it fits the HBM budget, else checkpointed (HR_HISTORY_REVOLVE).
+    Misfit and gradient are summed over the shots."""
+
+    def __init__(self, ctx: S.Context, survey: Survey, rho, params: InversionParams | None = None, wreg=None):
+        super().__init__(ctx, survey, rho, params, wreg, history=S.HR_HISTORY_REVOLVE)
+
+    def _shot(self, q, history):
+        sv = self.sv
+        sel = sv.src_shot == q
+        self.nforward += 1
+        return S.explicit_rk4_petsc_pml_3d(self.ctx, sv.nstep, sv.dt, sv.ftab[q], sv.src_eq[sel], sv.src_val[sel],
+                                           sv.rec_eq, history=history)
+
+    def observe(self, lam, mu):
+        self.ctx.set_material(Model(lam, mu, self.rho))
+        return np.stack([self._shot(q, S.HR_HISTORY_NONE).copy() for q in range(self.sv.ftab.shape[0])])
+
+    def forward_misfit(self, lam, mu):
+        sv = self.sv
+        self.ctx.set_material(Model(lam, mu, self.rho))
+        total = 0.0
+        for q in range(sv.ftab.shape[0]):
+            self._shot(q, S.HR_HISTORY_NONE)
+            total += S.compute_misfit(self.ctx, sv.nstep, sv.dt, sv.dis_meas[q])
+        return total
+
+    def gradient(self, it, lam, mu):
+        sv = self.sv
+        g_l = np.zeros(self.ctx.nnode)
+        g_m = np.zeros(self.ctx.nnode)
+        for q in range(sv.ftab.shape[0]):
+            self._shot(q, self.history)
+            gl, gm, _ = S.explicit_rk4_petsc_pml_3d_adjoint_otd(self.ctx, sv.nstep, sv.dt, sv.rec_eq, sv.dis_meas[q],
+                                                                with_misfit=False)
+            g_l += gl
+            g_m += gm
+        c_l, c_m, _, _ = S.regularize_gradient(self.ctx, it, g_l, g_m, lam, mu, self.P.regularization,
+                                               self.P.regularization_factor, self.wreg)
+        return g_l, g_m, c_l, c_m
+
+
+def survey_of(cfg, nstep=None):
+    """Survey of a config dict (mesh, src_nodes, rec_nodes, fr or frs, dt, nstep):
+    vertical unit point loads, Ricker time functions, shared receivers."""
+    from .mesh import receiver_dofs
+    mesh = cfg["mesh"]
+    nstep = int(nstep or cfg["nstep"])
+    nshot = len(cfg["src_nodes"])
+    frs = cfg.get("frs", [cfg.get("fr", 15.0)] * nshot)
+    ftab = np.stack([S.time_table(nstep, cfg["dt"], 2, float(fr)) for fr in frs])
+    return Survey(nstep=nstep, dt=cfg["dt"], ftab=ftab, src_shot=np.arange(nshot, dtype=np.int32),
+                  src_eq=np.array([mesh.dof(nd, 2) for nd in cfg["src_nodes"]], dtype=np.int64),
+                  src_val=np.ones(nshot), rec_eq=receiver_dofs(mesh, cfg["rec_nodes"]))
+
+
+def model_error(mesh, mu, mu_true):
+    """Relative L2 error of mu over the regular-domain node box."""
+    box = mesh.rd_node.reshape(-1)
+    d = mu[box] - mu_true[box]
+    return float(np.linalg.norm(d) / np.linalg.norm(mu_true[box]))

@@ -360,3 +360,71 @@ def config3(seed=3, nshot=16):
     src = np.array([mesh.surface_node(x, y) for y in sp for x in sp], dtype=np.int64)[:nshot]
     return dict(mesh=mesh, true=true, start=start, rec_nodes=rec, src_nodes=src, frs=frs, dt=1.0e-4,
                 nstep=3000)
+
+
+def ellipsoid_model(mesh: BoxMesh, vs_layers, rho_layers, interfaces, inclusions, vp_over_vs=2.0) -> Model:
+    """Layered model (as :func:`layered_model`) with ellipsoidal inclusions
+    ``(amp, (ax, ay, az), (x0, y0, z0))``: vs *= 1 + amp inside
+    ((x - x0)/ax)^2 + ((y - y0)/ay)^2 + ((z - z0)/az)^2 <= 1."""
+    base = layered_model(mesh, vs_layers, rho_layers, interfaces, vp_over_vs=vp_over_vs)
+    p = mesh.xyz()[mesh.material_map()]
+    z = p[:, 2]
+    rho = np.full(mesh.nnode, float(rho_layers[-1]))
+    vs = np.full(mesh.nnode, float(vs_layers[-1]))
+    for zi, r, v in zip(reversed(interfaces), reversed(rho_layers[:-1]), reversed(vs_layers[:-1])):
+        rho = np.where(z >= zi, float(r), rho)
+        vs = np.where(z >= zi, float(v), vs)
+    for amp, axes, c in inclusions:
+        q = (((p - np.asarray(c)) / np.asarray(axes)) ** 2).sum(axis=1)
+        vs = np.where(q <= 1.0, vs * (1.0 + amp), vs)
+    lam, mu, _ = vs_vp_rho_to_lame(vs, vp_over_vs * vs, rho)
+    return Model(lam, mu, base.rho)
+
+
+def gradient_model(mesh: BoxMesh, vs_top, vs_bottom, rho_layers, interfaces, vp_over_vs=2.0) -> Model:
+    """Vs linear in depth from vs_top at the surface to vs_bottom at the bottom
+    of the regular domain (constant below), densities of the layers."""
+    base = layered_model(mesh, (vs_top,) * len(rho_layers), rho_layers, interfaces, vp_over_vs=vp_over_vs)
+    p = mesh.xyz()[mesh.material_map()]
+    z = p[:, 2]
+    zb = mesh.zmin + mesh.rd_lo[2] * mesh.h[2]
+    t = np.clip(z / zb, 0.0, 1.0)
+    vs = vs_top + (vs_bottom - vs_top) * t
+    rho = np.full(mesh.nnode, float(rho_layers[-1]))
+    for zi, r in zip(reversed(interfaces), reversed(rho_layers[:-1])):
+        rho = np.where(z >= zi, float(r), rho)
+    lam, mu, _ = vs_vp_rho_to_lame(vs, vp_over_vs * vs, rho)
+    return Model(lam, mu, base.rho)
+
+
+def config4(seed=4, nx=96, ny=96, nz=48, npml=8, nrec_side=30, nshot_side=3):
+    """Full L-BFGS FWI: 96x96x48 elements (1 m), 8-element PML; true model of four
+    layers (Vs 150/220/300/400 m/s, interfaces z = -6, -14, -26 m, rho
+    1900/1950/2050/2100) with two ellipsoidal inclusions of +25 % and -25 % Vs
+    placed by default_rng(seed); starting model Vs linear from 150 m/s at the
+    surface to 450 m/s at the bottom of the regular domain; 9 shots on a 3x3
+    surface grid at 12 Hz (Ricker), 900 receivers on a 30x30 surface grid
+    spanning the regular domain, 5000 steps of 1e-4 s.  Smaller meshes keep the
+    same layout (scaled extents)."""
+    mesh = BoxMesh(nx, ny, nz, npml=(npml,) * 5)
+    rng = np.random.default_rng(seed)
+    hx = 0.5 * (nx - 2 * npml) * mesh.h[0]
+    hy = 0.5 * (ny - 2 * npml) * mesh.h[1]
+    depth = (nz - npml) * mesh.h[2]
+    f = depth / 40.0
+    layers = ((150.0, 220.0, 300.0, 400.0), (1900.0, 1950.0, 2050.0, 2100.0), (-6.0 * f, -14.0 * f, -26.0 * f))
+    inc = []
+    for amp in (0.25, -0.25):
+        axes = tuple(rng.uniform(0.08, 0.16, 3) * np.array([hx, hy, depth]))
+        c = (rng.uniform(-0.5, 0.5) * hx, rng.uniform(-0.5, 0.5) * hy, -rng.uniform(0.25, 0.7) * depth)
+        inc.append((amp, axes, c))
+    true = ellipsoid_model(mesh, *layers, inc)
+    start = gradient_model(mesh, 150.0, 450.0, layers[1], layers[2])
+    pts = np.round(np.linspace(-hx, hx, nrec_side) * 2.0) / 2.0
+    ptsy = np.round(np.linspace(-hy, hy, nrec_side) * 2.0) / 2.0
+    rec = np.array([mesh.surface_node(x, y) for y in ptsy for x in pts], dtype=np.int64)
+    sx = np.round(np.linspace(-0.5 * hx, 0.5 * hx, nshot_side) * 2.0) / 2.0
+    sy = np.round(np.linspace(-0.5 * hy, 0.5 * hy, nshot_side) * 2.0) / 2.0
+    src = np.array([mesh.surface_node(x, y) for y in sy for x in sx], dtype=np.int64)
+    return dict(mesh=mesh, true=true, start=start, rec_nodes=rec, src_nodes=src, fr=12.0, dt=1.0e-4,
+                nstep=5000, inclusions=inc)

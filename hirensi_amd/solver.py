@@ -199,6 +199,13 @@ class Context:
         self._check(self.L.hr_gradient_accumulate(self.h, _ptr(u), _ptr(p), float(scale), _ptr(g_lambda),
                                                   _ptr(g_mu)))
 
+    def autotune(self, nshot=1, steps=6):
+        """Pick the stage-kernel z-chunk length by timing a few RK4 steps per
+        candidate (``hr_autotune``); returns (tz, ms per step)."""
+        tz, ms = C.c_int(), C.c_double()
+        self._check(self.L.hr_autotune(self.h, int(nshot), int(steps), C.byref(tz), C.byref(ms)))
+        return tz.value, ms.value
+
     def set_history_budget(self, nbytes):
         """HBM bytes for the HR_HISTORY_REVOLVE history (0: half the free memory)."""
         self._check(self.L.hr_set_history_budget(self.h, float(nbytes)))

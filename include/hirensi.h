@@ -254,6 +254,13 @@ int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version);
 int hr_set_profiling(hr_ctx* ctx, int mode);
 int hr_get_kernel_stats(hr_ctx* ctx, long long* launches, double* ms);
 
+/* Autotuner of the stage-kernel tiling: times `steps` RK4 steps of nshot shots
+ * (zero state, no sources: the same launches and traffic) for each z-chunk
+ * length of the tiles (3..12 element layers) and keeps the fastest; returns it
+ * and its time per step (ms).  Invalidates the history of the last forward.
+ * No counterpart in the reference (a PETSc AIJ product has no tiling). */
+int hr_autotune(hr_ctx* ctx, int nshot, int steps, int* tz_best, double* ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif
