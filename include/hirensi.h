@@ -143,6 +143,17 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
 /* Compute_Misfit of the traces of the last forward against dis_meas (summed over its shots). */
 int hr_misfit(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_meas, double* misfit);
 
+/* hr_misfit without allocation: the same kernels and summation order (the
+ * result equals hr_misfit's bit for bit), with the per-step partial sums in a
+ * fixed device array of 1048576 doubles (8 MiB per GPU, held until the process
+ * ends) instead of memory the library allocates.  No counterpart in the
+ * reference: it is for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  Needs a context
+ * created with HR_DEVICE_PTRS (dis_meas and misfit on the device); refuses
+ * nshot x (nstep + 1) + nshot > 1048576 with an error.  All calls on one GPU
+ * share the array: calls that overlap on one GPU race. */
+int hr_misfit_noalloc(hr_ctx* ctx, int nstep, double dt, int nrec, const double* dis_meas, double* misfit);
+
 /* Update_g (Assemble_PETSc_3D.F90:556-661) for one step of the control pass
  * (Solver_PETSc_Explicit.F90:1474-1510): over the regular-domain elements
  * (MTYPE 1), g_lambda += scale * W div u div p and g_mu += scale * W 2 eps(u):eps(p)
