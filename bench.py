@@ -49,6 +49,17 @@ def parse():
     p.add_argument("--fwi-iters", type=int, default=20)
     p.add_argument("--fwi-variant", choices=["both", "batched", "plain"], default="both")
     p.add_argument("--fwi-budget-s", type=float, default=900.0, help="stop a variant after this many seconds")
+    p.add_argument("--precond", action="store_true",
+                   help="instead of the default line: train the learned gradient preconditioner on solver-made "
+                        "pairs (32x32x16) and run L-BFGS with and without it on a 64x64x32 survey")
+    p.add_argument("--pc-pairs", type=int, default=200, help="training pairs")
+    p.add_argument("--pc-nstep", type=int, default=900, help="RK4 steps of the training solves")
+    p.add_argument("--pc-epochs", type=int, default=60)
+    p.add_argument("--pc-fwi-nstep", type=int, default=1800, help="RK4 steps of the inversion solves")
+    p.add_argument("--pc-fwi-iters", type=int, default=8)
+    p.add_argument("--pc-seed", type=int, default=5, help="seed of the inversion's true model")
+    p.add_argument("--pc-weights", default="", help="load trained weights (.npz) instead of training")
+    p.add_argument("--pc-save", default="", help="save the trained weights (.npz) here")
     p.add_argument("--fd-check", action="store_true",
                    help="instead of timing: check the lambda/mu gradient against central finite differences "
                         "along 5 seeded smooth directions on this mesh (use with --nstep)")
@@ -350,8 +361,100 @@ def fwi_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def precond_bench(a):
+    """Learned gradient preconditioner: training pairs from the FP64 solver on
+    random layered models with Gaussian inclusions (32x32x16 elements, PML 4, 4
+    shots at 30 Hz), training with the HIP kernels (BF16 weights, FP32
+    accumulation), then L-BFGS (m = 5) on a 64x64x32 survey of the same family
+    with the plain H0 and with the learned one.  Reports the training cost, the
+    held-out cosine of raw gradient vs network output with the true error, the
+    misfit and mu error per iteration, and the iterations and wall time of the
+    learned run to reach the plain run's final misfit.  Prints one JSON line."""
+    import numpy as np
+    from hirensi_amd import fwi as F
+    from hirensi_amd import precond as PC
+    from hirensi_amd import solver as S
+    from hirensi_amd.mesh import BoxMesh
+    out = {"metric": "learned-preconditioned L-BFGS: wall time to the plain run's final misfit (64x64x32)",
+           "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64 solver, bf16/f32 network",
+           "data": "synthetic (random layered models with 1-3 Gaussian Vs inclusions)",
+           "network": {"layers": len(PC.LAYERS), "params": PC.n_params()}}
+    net = PC.ConvNet(seed=0)
+    if a.pc_weights:
+        net.load(a.pc_weights)
+        out["training"] = {"weights": a.pc_weights}
+    else:
+        t0 = time.perf_counter()
+        X, T = PC.generate_pairs(a.pc_pairs, seed=1, nstep=a.pc_nstep, log=_log)
+        t_gen = time.perf_counter() - t0
+        nval = max(4, a.pc_pairs // 10)
+        t1 = time.perf_counter()
+        hist = PC.train(net, X[nval:], T[nval:], epochs=a.pc_epochs, batch=8, lr=1e-3, log=_log, log_every=10)
+        import torch
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t1
+        raw_cos, nn_cos = PC.evaluate(net, X[:nval], T[:nval])
+        tr_raw, tr_nn = PC.evaluate(net, X[nval:nval + 20], T[nval:nval + 20])
+        _log(f"precond: held-out cosine with the true error: raw gradient {raw_cos:.3f}, network {nn_cos:.3f}")
+        out["training"] = {"pairs": a.pc_pairs, "held_out": nval, "pair_nstep": a.pc_nstep,
+                           "pair_generation_s": t_gen, "train_s": t_train, "epochs": a.pc_epochs,
+                           "loss_first": hist[0], "loss_last": hist[-1], "loss_history": hist,
+                           "held_out_cosine_raw": raw_cos, "held_out_cosine_net": nn_cos,
+                           "train_cosine_raw": tr_raw, "train_cosine_net": tr_nn}
+        if a.pc_save:
+            net.save(a.pc_save)
+    mesh = BoxMesh(64, 64, 32, npml=(6,) * 5)
+    true, start = PC.random_case(mesh, np.random.default_rng(a.pc_seed))
+    sv = PC.training_survey(mesh, a.pc_fwi_nstep, nshot_side=2, nrec_side=20)
+    ctx = S.Context(mesh)
+    _log(f"precond: 64x64x32 survey, {sv.ftab.shape[0]} shots, {a.pc_fwi_nstep} steps; observing the true model")
+    sv.dis_meas = F.HipProblem(ctx, sv, true.rho).observe(true.lam, true.mu)
+    _, restriction = S.gradient_mass(ctx)
+    params = F.InversionParams(m_lbfgs=5, bound_lambda_lower=40.0, bound_lambda_upper=1200.0, bound_mu_lower=20.0,
+                               bound_mu_upper=600.0, fac_max_update_lambda=10.0, fac_max_update_mu=10.0)
+    runs = {}
+    for v in ("plain", "learned"):
+        prob = F.HipProblem(ctx, sv, start.rho, params)
+        h0 = PC.LearnedH0(net, mesh, restriction) if v == "learned" else None
+        t0 = time.perf_counter()
+        inv = F.Inversion(prob, start.lam, start.mu, restriction, params, h0_map=h0)
+        recs = [{"iter": 0, "misfit": inv.misfit, "mu_error": F.model_error(mesh, inv.mu, true.mu),
+                 "wall_s": time.perf_counter() - t0}]
+        stop = ""
+        for _ in range(a.pc_fwi_iters):
+            try:
+                r = inv.iterate()
+            except F.ArmijoFailure as e:
+                stop = str(e)
+                break
+            recs.append({"iter": r["iter"], "misfit": r["misfit"], "mu_error": F.model_error(mesh, inv.mu, true.mu),
+                         "armijo_cuts": r["armijo_counter"], "learned_h0": r.get("learned_h0"),
+                         "fallback": r.get("fallback"), "wall_s": time.perf_counter() - t0})
+            _log(f"precond {v}: iteration {r['iter']} misfit {r['misfit']:.6e} mu error {recs[-1]['mu_error']:.4f}"
+                 f"{' (learned step)' if r.get('learned_h0') else ''} ({recs[-1]['wall_s']:.1f} s)")
+        runs[v] = {"records": recs, "forward_solves": prob.nforward, "stopped": stop}
+        if h0 is not None:
+            runs[v]["h0_calls"], runs[v]["h0_used"] = h0.calls, h0.used
+    ctx.close()
+    target = runs["plain"]["records"][-1]["misfit"]
+    hit = [r for r in runs["learned"]["records"] if r["misfit"] <= target]
+    out["config"] = {"workload": "64x64x32 el, PML 6, 4 shots at 30 Hz, 400 receivers", "nstep": a.pc_fwi_nstep,
+                     "iterations": a.pc_fwi_iters, "true_model_seed": a.pc_seed}
+    out["runs"] = runs
+    out["plain_final_misfit"] = target
+    out["plain_time_s"] = runs["plain"]["records"][-1]["wall_s"]
+    out["value"] = hit[0]["wall_s"] if hit else None
+    out["learned_iterations_to_plain_misfit"] = hit[0]["iter"] if hit else None
+    if hit:
+        out["speedup"] = out["plain_time_s"] / hit[0]["wall_s"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
+    if a.precond:
+        precond_bench(a)
+        return
     if a.fwi:
         fwi_bench(a)
         return

@@ -35,7 +35,7 @@ def _run(cmd, cwd=None):
 
 def build_hip(extra=None):
     srcs = [os.path.join(CSRC, f) for f in ("hr_kernels.hip", "hr_stage3.hip", "hr_reg.hip", "hr_api.hip",
-                                            "hr_revolve.cpp", "hr_noalloc.hip")]
+                                            "hr_revolve.cpp", "hr_noalloc.hip", "hr_nn.hip")]
     tmp = LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, *(extra or []), "-o", tmp, *srcs])
     os.replace(tmp, LIB)

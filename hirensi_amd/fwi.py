@@ -76,14 +76,23 @@ class CoupledLBFGS:
     over lambda and mu.  The stored s is restricted to the regular-domain nodes;
     rho and H0 use the unrestricted s (LBFGS_Coupled.F90:13-215)."""
 
-    def __init__(self, m):
+    def __init__(self, m, h0_map=None):
         self.pairs = deque(maxlen=m)     # (s_l, s_m, y_l, y_m, rho), newest at the right
+        self.h0_map = h0_map             # q_mu -> P(q_mu) or None (learned H0; None: identity)
+        self.h0 = None
+        self.mapped = False
 
-    def direction(self, g_l, g_m, d_g_l, d_g_m, d_l, d_m, restriction):
+    def direction(self, g_l, g_m, d_g_l, d_g_m, d_l, d_m, restriction, use_map=True):
         """g: current gradient; d_g = g - g_pre; d = m - m_pre.  Returns p = -H g."""
         ys = _dot(d_g_l, d_l) + _dot(d_g_m, d_m)
         yy = _dot(d_g_l, d_g_l) + _dot(d_g_m, d_g_m)
         self.pairs.append((d_l * restriction, d_m * restriction, d_g_l, d_g_m, 1.0 / ys))
+        self.h0 = ys / yy
+        return self.two_loop(g_l, g_m, use_map)
+
+    def two_loop(self, g_l, g_m, use_map=True):
+        """p = -H g over the stored pairs with H0 = h0 I, or h0 P on the mu
+        field when a learned map is set, used and returns a direction."""
         q_l, q_m = _copy(g_l), _copy(g_m)
         alphas = []
         for s_l, s_m, y_l, y_m, rho in reversed(self.pairs):
@@ -91,8 +100,14 @@ class CoupledLBFGS:
             alphas.append(a)
             q_l = q_l - a * y_l
             q_m = q_m - a * y_m
-        h0 = ys / yy
+        h0 = self.h0
         r_l, r_m = h0 * q_l, h0 * q_m
+        self.mapped = False
+        if use_map and self.h0_map is not None:
+            pq = self.h0_map(q_m)
+            if pq is not None:
+                r_m = h0 * pq
+                self.mapped = True
         for (s_l, s_m, y_l, y_m, rho), a in zip(self.pairs, reversed(alphas)):
             b = rho * _dot(r_l, y_l) + rho * _dot(r_m, y_m)
             r_l = r_l + s_l * (a - b)
@@ -103,7 +118,8 @@ class CoupledLBFGS:
 class Inversion:
     """State of one inversion run (Program_Main.F90:379-921)."""
 
-    def __init__(self, problem, lam, mu, restriction, params: InversionParams | None = None, it_begin=1):
+    def __init__(self, problem, lam, mu, restriction, params: InversionParams | None = None, it_begin=1,
+                 h0_map=None):
         self.problem = problem
         self.P = params or InversionParams()
         self.lam, self.mu = _copy(lam), _copy(mu)
@@ -111,7 +127,12 @@ class Inversion:
         self.it = it_begin
         self.it_begin = it_begin
         self.fac_max = [self.P.fac_max_update_lambda, self.P.fac_max_update_mu]
-        self.lbfgs = CoupledLBFGS(self.P.m_lbfgs)
+        # h0_map: learned stand-in for the identity in H0 (mu field), applied to
+        # the L-BFGS and first (steepest-descent) directions; the plain direction
+        # is the fallback when the line search rejects the first trial step
+        self.h0_map = h0_map
+        self.mapped = False
+        self.lbfgs = CoupledLBFGS(self.P.m_lbfgs, h0_map)
         self.g = None
         self.p = None
         self.history = []
@@ -141,9 +162,15 @@ class Inversion:
         return ((c * abs(a_m) * gs_m if cp != "Lambda" else 0.0)
                 + (c * abs(a_l) * gs_l if cp != "Mu" else 0.0))
 
-    def _direction(self, g_l, g_m):
+    def _direction(self, g_l, g_m, use_map=True):
         P = self.P
+        self.mapped = False
         if P.search_direction == "SD" or self.it == self.it_begin:
+            if use_map and self.h0_map is not None:
+                pg = self.h0_map(g_m)
+                if pg is not None:
+                    self.mapped = True
+                    return -g_l, -pg
             return -g_l, -g_m
         if P.search_direction == "CG":
             if self.it % P.n_cg_restart == 0:
@@ -152,18 +179,46 @@ class Inversion:
             b_l = math.sqrt(_dot(g_l, g_l)) / math.sqrt(_dot(gl0, gl0))
             b_m = math.sqrt(_dot(g_m, g_m)) / math.sqrt(_dot(gm0, gm0))
             return -g_l + b_l * pl0, -g_m + b_m * pm0
+        if not use_map:          # the fallback of a rejected learned step: the pairs are already stored
+            return self.lbfgs.two_loop(g_l, g_m, use_map=False)
         gl0, gm0 = self.g
-        return self.lbfgs.direction(g_l, g_m, g_l - gl0, g_m - gm0, self.lam - self.lam_pre,
-                                    self.mu - self.mu_pre, self.restriction)
+        p = self.lbfgs.direction(g_l, g_m, g_l - gl0, g_m - gm0, self.lam - self.lam_pre,
+                                 self.mu - self.mu_pre, self.restriction)
+        self.mapped = self.lbfgs.mapped
+        return p
 
     # ------------------------------------------------------------ iteration
     def iterate(self):
         """One inversion iteration; returns its record."""
-        P = self.P
         misfit_pre = self.misfit
         g_l, g_m, c_rl, c_rm = self.problem.gradient(self.it, self.lam, self.mu)
         p_l, p_m = self._direction(g_l, g_m)
+        mapped = self.mapped
         self.lam_pre, self.mu_pre = self.lam, self.mu
+        res =self._line_search(g_l, g_m, p_l, p_m, misfit_pre, first_only=mapped)
+        fallback = False
+        if res is None:
+            # the learned direction's first trial was rejected: plain direction
+            fallback = True
+            p_l, p_m = self._direction(g_l, g_m, use_map=False)
+            res = self._line_search(g_l, g_m, p_l, p_m, misfit_pre)
+        lam, mu, misfit, a, cuts, p_l, p_m = res
+        self.lam, self.mu, self.misfit = lam, mu, misfit
+        self.g, self.p = (g_l, g_m), (p_l, p_m)
+        rec = {"iter": self.it, "misfit": misfit, "misfit_pre": misfit_pre, "cost_reg": (c_rl, c_rm),
+               "alpha": (a[0], a[1]), "armijo_counter": cuts}
+        if self.h0_map is not None:
+            rec["learned_h0"] = mapped and not fallback
+            rec["fallback"] = fallback
+        self.history.append(rec)
+        self.it += 1
+        return rec
+
+    def _line_search(self, g_l, g_m, p_l, p_m, misfit_pre, first_only=False):
+        """Descent check, step lengths and Armijo back-tracking of one direction;
+        first_only: return None if the direction fails the descent check or its
+        first trial step is rejected."""
+        P = self.P
         sd = [False, False]
         gs = [_dot(g_l, p_l), _dot(g_m, p_m)]
         if gs[0] > 0:
@@ -172,6 +227,8 @@ class Inversion:
         if gs[1] > 0:
             sd[1], p_m = True, -g_m
             gs[1] = _dot(g_m, p_m)
+        if first_only and (sd[0] or sd[1]):
+            return None          # a learned direction that is not a descent direction: fall back
         pmax = [_absmax(p_l), _absmax(p_m)]
         a_sd = [self.fac_max[0] / pmax[0], self.fac_max[1] / pmax[1]]
         a = list(a_sd)
@@ -181,6 +238,8 @@ class Inversion:
         lam, mu = self._step(self.lam_pre, self.mu_pre, p_l, p_m, a[0], a[1])
         misfit = self.problem.forward_misfit(lam, mu)
         armijo = self._armijo(a[0], a[1], gs[0], gs[1])
+        if first_only and misfit > misfit_pre + armijo:
+            return None
         cuts = 0
         while misfit > misfit_pre + armijo:
             cuts += 1
@@ -194,18 +253,12 @@ class Inversion:
             armijo = self._armijo(a[0], a[1], gs[0], gs[1])
             if cuts == 10:
                 raise ArmijoFailure(f"iteration {self.it}: no sufficient decrease after 10 cuts")
-        self.lam, self.mu, self.misfit = lam, mu, misfit
-        self.g, self.p = (g_l, g_m), (p_l, p_m)
-        rec = {"iter": self.it, "misfit": misfit, "misfit_pre": misfit_pre, "cost_reg": (c_rl, c_rm),
-               "alpha": (a[0], a[1]), "armijo_counter": cuts}
-        self.history.append(rec)
-        self.it += 1
-        return rec
+        return lam, mu, misfit, a, cuts, p_l, p_m
 
 
-def invert(problem, lam, mu, restriction, n_iter, params: InversionParams | None = None, it_begin=1):
+def invert(problem, lam, mu, restriction, n_iter, params: InversionParams | None = None, it_begin=1, h0_map=None):
     """Run n_iter iterations from (lam, mu); returns (lam, mu, records)."""
-    inv = Inversion(problem, lam, mu, restriction, params, it_begin)
+    inv = Inversion(problem, lam, mu, restriction, params, it_begin, h0_map=h0_map)
     for _ in range(n_iter):
         inv.iterate()
     return inv.lam, inv.mu, inv.history

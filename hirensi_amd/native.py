@@ -27,6 +27,13 @@ EXPORTS = (
     "hr_autotune", "hr_misfit_noalloc",
 )
 
+# include/hirensi_nn.h (signatures set by hirensi_amd.precond.lib)
+NN_EXPORTS = (
+    "hr_nn_conv3d", "hr_nn_conv3d_wgrad", "hr_nn_transpose_weights", "hr_nn_leaky_relu", "hr_nn_leaky_relu_grad",
+    "hr_nn_avgpool2", "hr_nn_avgpool2_grad", "hr_nn_upsample2", "hr_nn_upsample2_grad", "hr_nn_mse", "hr_nn_adam",
+    "hr_nn_to_bf16",
+)
+
 
 class MeshDesc(C.Structure):
     _fields_ = [

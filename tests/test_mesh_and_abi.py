@@ -95,8 +95,8 @@ def test_configs_build():
     assert abs(np.sqrt(mu[n] / rho) - layer * (1 - 0.3 * np.exp(-r2 / 16.0))) < 1e-9
 
 
-def _header_functions():
-    txt = open(os.path.join(ROOT, "include", "hirensi.h")).read()
+def _header_functions(name="hirensi.h"):
+    txt = open(os.path.join(ROOT, "include", name)).read()
     return sorted(set(re.findall(r"^\s*(?:int|double|const char\*)\s+(hr_\w+)\s*\(", txt, re.M)))
 
 
@@ -108,6 +108,12 @@ def test_cabi_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(native.EXPORTS)
+    nn = _header_functions("hirensi_nn.h")
+    assert len(nn) == 12
+    for n in nn:
+        assert hasattr(lib, n), n
+    assert set(nn) == set(native.NN_EXPORTS)
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["hirensi.h", "hirensi_nn.h"]
 
 
 def test_host_time_function_matches_oracle():
