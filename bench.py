@@ -38,6 +38,9 @@ def parse():
     p.add_argument("--ckpt-budget", type=float, default=8.0e9,
                    help="HBM budget (bytes) of the checkpointed leg (binomial revolve history); 0 skips the leg")
     p.add_argument("--cpu-steps", type=int, default=40, help="oracle RK4 steps timed (each direction)")
+    p.add_argument("--graph-leg", type=int, default=8,
+                   help="also time the gradient with time loops replayed from hipGraphs of this many steps "
+                        "(0 skips the leg)")
     p.add_argument("--copy-calib", action="store_true", help="also run a plain 1 GiB device copy (PMC scale)")
     p.add_argument("--shots", type=int, default=0,
                    help="instead of the default line: batched multi-shot gradient on the 48x48x24 survey "
@@ -582,6 +585,28 @@ def main():
                      "fused_achieved": fused_bytes / (kern_ms * 1e-3) / 1e9,
                      "stage_kernel_share_of_wall": kern_ms * 1e-3 / (wall * a.steps)},
     }
+    if a.graph_leg > 0:
+        # the same gradient with its time loops replayed from a captured hipGraph of
+        # --graph-leg steps, against eager launches, both without timing events
+        L.hr_set_profiling(ctx.h, 0)
+        g_ref = d_gm.clone()
+        walls = {}
+        for mode, k in (("eager", 0), ("graph", a.graph_leg)):
+            ctx.set_graph(k)
+            gradient()
+            torch.cuda.synchronize()
+            t0g = time.perf_counter()
+            for _ in range(2):
+                gradient()
+            torch.cuda.synchronize()
+            walls[mode] = (time.perf_counter() - t0g) / 2
+            _log(f"{mode} gradient without timing events: {walls[mode]:.3f} s")
+        same = bool(torch.equal(d_gm, g_ref))
+        ctx.set_graph(0)
+        out["graph"] = {"steps_per_graph": a.graph_leg, "graph_wall_s": walls["graph"],
+                        "eager_wall_s": walls["eager"], "speedup": walls["eager"] / walls["graph"],
+                        "value": dof_steps / walls["graph"], "graph_launches": ctx.graph_launches(),
+                        "gradient_bitwise_equal_to_eager": same}
     if a.ckpt_budget > 0:
         # the same gradient with the forward history checkpointed in HBM within a budget
         # (binomial revolve schedule); one warm-up (allocates the snapshot pool), one timed

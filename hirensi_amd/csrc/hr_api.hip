@@ -34,6 +34,7 @@ struct HipError {
 struct Src3Spec {
   const Src3* lists = nullptr;   // host copy of the solve's source lists
   int mode = 0, ra = 0, rb = -1, step = 0, fcol = 0;
+  bool rb_on = false;            // the second adjoint row rb is used
   double ca = 0.0, cb = 0.0;
 };
 
@@ -151,6 +152,14 @@ struct hr_ctx {
   int ntl[3] = {0, 0, 0};
   // stage-kernel timing
   int profiling = 0;
+  // captured time loops: graph_k steps per hipGraph (0: eager launches), the
+  // device-side step index they read, and whether launches are being captured
+  int graph_k = 0;
+  int* d_step = nullptr;
+  int dyn = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long long graph_launches = 0;
   std::vector<hipEvent_t> ev_pool;
   // one launch in PROF_PERIOD is timed: (start, end) events around the kernel on
   // each stream it uses; its span is the first start to the last end
@@ -383,7 +392,9 @@ Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
   B.hh = A.hh; B.h = A.h; B.h6 = A.h6;
   if (sp && sp->mode) {
     B.src = *sp->lists; B.src_mode = sp->mode; B.ra = sp->ra; B.rb = sp->rb; B.ca = sp->ca; B.cb = sp->cb;
+    B.rb_on = sp->rb_on ? 1 : 0;
     B.step = sp->step; B.fcol = sp->fcol;
+    B.step_dev = ctx->dyn ? ctx->d_step : nullptr;
   }
   // stage launches over the state buffers cover every shot; hr_apply works on one field
   B.nshot = A.stage > 0 ? ctx->nshot : 1;
@@ -560,18 +571,20 @@ int rk4_step(hr_ctx* ctx, double dt, int dir, int transpose, const double sgn[3]
 int record_step(hr_ctx* ctx, int n) {
   if (ctx->f_nrec <= 0) return 0;
   const size_t nf = field_size(ctx), rows = (size_t)ctx->f_nstep + 1;
+  const StepRef sr{ctx->dyn ? ctx->d_step : nullptr, ctx->f_nrec};
   for (int q = 0; q < ctx->nshot; ++q)
     HIPC(launch_record(ctx->f_nrec, ctx->d_rec_slot, ctx->x[1] + q * nf,
-                       ctx->d_traces + ((size_t)q * rows + n) * ctx->f_nrec, ctx->stream));
+                       ctx->d_traces + ((size_t)q * rows + n) * ctx->f_nrec, ctx->stream, sr));
   return 0;
 }
 
 // regular-domain displacement of every shot into history row `row` of blocks of `rows` rows per shot
 int store_slice(hr_ctx* ctx, double* base, size_t rows, size_t row) {
   const size_t nf = field_size(ctx), slice = 3 * (size_t)ctx->nbox;
+  const StepRef sr{ctx->dyn ? ctx->d_step : nullptr, (long long)slice};
   for (int q = 0; q < ctx->nshot; ++q)
     HIPC(launch_store_box(ctx->g, ctx->box0, ctx->boxn, ctx->x[1] + q * nf, base + ((size_t)q * rows + row) * slice,
-                          ctx->stream));
+                          ctx->stream, sr));
   return 0;
 }
 
@@ -585,6 +598,56 @@ void forward_sources(hr_ctx* ctx, int n, Src3Spec sp4[4]) {
     sp4[s].step = n;
     sp4[s].fcol = (s == 0) ? 0 : (s == 3 ? 2 : 1);
   }
+}
+
+// Steps start, start + dir, ... (count of them) of a time loop whose body(n)
+// queues step n.  Eager, or (graph_k > 0) through a hipGraph of graph_k steps
+// captured once and replayed: the per-step values (source row, trace and history
+// rows) then come from a device-side step index that each captured step advances
+// by dir, the body is captured with n = 0 (ctx->dyn: n is an offset from that
+// index), and the steps that do not fill a graph run eagerly.  graph_k is even,
+// so the double-buffered state has the same parity at every replay.
+int run_steps(hr_ctx* ctx, int start, int count, int dir, const std::function<int(int)>& body) {
+  const int K = ctx->graph_k;
+  int done = 0;
+  if (K > 0 && count >= K && !ctx->profiling) {
+    if (ctx->gexec || ctx->graph) {
+      HIPC(hipStreamSynchronize(ctx->stream));
+      if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+      if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+      ctx->gexec = nullptr;
+      ctx->graph = nullptr;
+    }
+    if (!ctx->d_step) if (int rc = dalloc(ctx, &ctx->d_step, 1)) return rc;
+    HIPC(launch_step_set(ctx->d_step, start, ctx->stream));
+    HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->dyn = 1;
+    int rc = 0;
+    for (int k = 0; k < K && !rc; ++k) {
+      rc = body(0);
+      if (!rc && launch_step_advance(ctx->d_step, dir, ctx->stream) != hipSuccess)
+        rc = fail(ctx, 3, "step index launch failed in capture");
+    }
+    ctx->dyn = 0;
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (rc || ec != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      ctx->deferred.clear();
+      return rc ? rc : fail(ctx, 3, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+    }
+    ctx->graph = g;
+    HIPC(hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
+    const int reps = count / K;
+    for (int r = 0; r < reps; ++r) {
+      HIPC(hipGraphLaunch(ctx->gexec, ctx->stream));
+      ctx->graph_launches += 1;
+    }
+    done = reps * K;
+  }
+  for (int i = done; i < count; ++i)
+    if (int rc = body(start + dir * i)) return rc;
+  return 0;
 }
 
 int run_forward_steps(hr_ctx* ctx, int n0, int n1, double* seg_hist, size_t seg_rows) {
@@ -774,6 +837,9 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   F(ctx->d_img); F(ctx->d_restr); F(ctx->d_reg); F(ctx->d_cost_el); F(ctx->d_red); F(ctx->d_pm); F(ctx->d_pg);
   for (int c = 0; c < 3; ++c) F(ctx->d_tl[c]);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  F(ctx->d_step);
   for (auto& p : ctx->ev_open)
     for (int q = 0; q < p.n; ++q) { (void)hipEventDestroy(p.start[q]); (void)hipEventDestroy(p.end[q]); }
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -1008,6 +1074,19 @@ int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   g.ty = t;
   g.tz = tz ? tz : DEFAULT_TZ;
   return setup_tiles(ctx);
+}
+
+int hr_set_graph(hr_ctx* ctx, int steps) {
+  if (!ctx) return 1;
+  if (steps < 0 || (steps & 1) || steps > 4096) return fail(ctx, 1, "steps per graph must be 0 or an even count <= 4096");
+  ctx->graph_k = steps;
+  return 0;
+}
+
+int hr_get_graph_launches(hr_ctx* ctx, long long* launches) {
+  if (!ctx || !launches) return 1;
+  *launches = ctx->graph_launches;
+  return 0;
 }
 
 }  // extern "C"
@@ -1312,7 +1391,7 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
                          ctx->d_energy + (size_t)q * ctx->e_rows + n / ctx->e_every, ctx->stream));
     return 0;
   };
-  for (int n = 0; n < nstep; ++n) {
+  auto step_body = [&](int n) -> int {
     // traces and history slice of the state at the start of step n, queued beside its stage 1
     ctx->deferred.push_back([ctx, n]() { return record_step(ctx, n); });
     if (ctx->e_rows && n % ctx->e_every == 0) ctx->deferred.push_back([energy_at, n]() { return energy_at(n); });
@@ -1328,7 +1407,14 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
     }
     Src3Spec sp4[4];
     forward_sources(ctx, n, sp4);
-    if (int rc = rk4_step(ctx, dt, +1, 0, sgn, sp4)) return rc;
+    return rk4_step(ctx, dt, +1, 0, sgn, sp4);
+  };
+  if (L == 0 && ctx->e_rows == 0) {
+    // per-step work depends on n only through the device step index: capturable
+    if (int rc = run_steps(ctx, 0, nstep, +1, step_body)) return rc;
+  } else {
+    for (int n = 0; n < nstep; ++n)
+      if (int rc = step_body(n)) return rc;
   }
   if (L > 0) if (int rc = store_slice(ctx, ctx->d_seg, seg_rows, (size_t)(nstep - last0))) return rc;
   if (int rc = record_step(ctx, nstep)) return rc;
@@ -1539,10 +1625,11 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     if (n <= nstep - 1) {
       // queued beside stage 1 of the adjoint step, which reads but does not write p
       double* const a1 = P[1];
-      ctx->deferred.push_back([ctx, nshot, uh, uh_rows, uh_row, slice, a1, nf, dt]() -> int {
+      const StepRef sr{ctx->dyn ? ctx->d_step : nullptr, (long long)slice};
+      ctx->deferred.push_back([ctx, nshot, uh, uh_rows, uh_row, slice, a1, nf, dt, sr]() -> int {
         for (int q = 0; q < nshot; ++q)
           HIPC(launch_gradient(ctx->g, ctx->box0, ctx->boxn, uh + ((size_t)q * uh_rows + uh_row) * slice,
-                               a1 + q * nf, dt, ctx->d_gl, ctx->d_gm, ctx->stream));
+                               a1 + q * nf, dt, ctx->d_gl, ctx->d_gm, ctx->stream, sr));
         return 0;
       });
     }
@@ -1555,7 +1642,7 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
       sp4[st].mode = 2;
       if (st == 0) { sp4[st].ra = n; sp4[st].ca = 1.0; }
       else if (st == 3) { sp4[st].ra = n - 1; sp4[st].ca = 1.0; }
-      else { sp4[st].ra = n; sp4[st].ca = 0.5; sp4[st].rb = n - 1; sp4[st].cb = 0.5; }
+      else { sp4[st].ra = n; sp4[st].ca = 0.5; sp4[st].rb = n - 1; sp4[st].cb = 0.5; sp4[st].rb_on = true; }
     }
     use(P, PY);
     const int rc = rk4_step(ctx, dt, -1, 1, sgn, sp4);
@@ -1565,7 +1652,13 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
   long long recomputed = 0;
   int status = 0;
   if (ctx->h_store) {
-    for (int n = nstep; n >= 1 && !status; --n) status = adjoint_step(n, ctx->d_hist, rows, (size_t)n);
+    // step nstep has no gradient term; steps nstep-1 .. 1 are alike (capturable):
+    // in a captured step n is the offset 0 from the device step index
+    status = adjoint_step(nstep, ctx->d_hist, rows, (size_t)nstep);
+    if (!status && nstep > 1)
+      status = run_steps(ctx, nstep - 1, nstep - 1, -1, [&](int n) -> int {
+        return adjoint_step(ctx->dyn ? 0 : n, ctx->d_hist, rows, (size_t)(ctx->dyn ? 0 : n));
+      });
   } else {
     // the reverse sweep of the binomial schedule; the forward kept the slices of the last segment
     const std::vector<RevAction> acts = ctx->h_plan.actions(true);

@@ -45,11 +45,21 @@ hipError_t launch_gather_eq(long long neq, const long long* slot, const double* 
                             hipStream_t s);
 hipError_t launch_scatter_eq(long long neq, const long long* slot, const double* raw, const double* y, double sc,
                              const double* l, double* z, hipStream_t s);
-hipError_t launch_record(int nrec, const long long* slot, const double* x2, double* row, hipStream_t s);
+// Device-side step index of a captured time loop: the kernel reads *step once and
+// offsets its per-step pointer by (*step) * stride elements (step null: no offset).
+struct StepRef {
+  const int* step = nullptr;
+  long long stride = 0;
+};
+hipError_t launch_record(int nrec, const long long* slot, const double* x2, double* row, hipStream_t s,
+                         StepRef sr = StepRef());
 hipError_t launch_store_box(const Geo& g, const int* box0, const int* boxn, const double* x2, double* hist,
-                            hipStream_t s);
+                            hipStream_t s, StepRef sr = StepRef());
 hipError_t launch_gradient(const Geo& g, const int* box0, const int* boxn, const double* uh, const double* p2,
-                           double dt, double* glam, double* gmu, hipStream_t s);
+                           double dt, double* glam, double* gmu, hipStream_t s, StepRef sr = StepRef());
+// *step = value; *step += delta (one lane, a vector store)
+hipError_t launch_step_set(int* step, int value, hipStream_t s);
+hipError_t launch_step_advance(int* step, int delta, hipStream_t s);
 hipError_t launch_gradient_finish(const Geo& g, const int* box0, const int* boxn, const double* glb,
                                   const double* gmb, double* gl, double* gm, hipStream_t s);
 hipError_t launch_permute_cols(long long nrows, int ncols, const long long* perm, const double* src, double* dst,
@@ -128,8 +138,9 @@ struct Stage3Args {
   int set_mask;                    // apply: bit0 G, bit1 K, bit2 C
   int src_mode;                    // 0 none, 1 forward: f * B * m, 2 adjoint misfit
   int step, fcol;                  // forward: f = ftab[(shot * nstep + step) * 3 + fcol]
-  int ra, rb;                      // adjoint source rows (rb < 0: none)
+  int ra, rb, rb_on;               // adjoint source rows (rb used when rb_on)
   double ca, cb;                   // adjoint row weights
+  const int* step_dev;             // captured loops: step, ra, rb are offsets from *step_dev
   int nshot;                       // shots in the launch; state vectors are shot-major
   long long shot_stride;           // elements per shot of a state vector (9 NN)
   double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed); stage 5: dt^2, dt, dt^3

@@ -44,22 +44,36 @@ __global__ void scatter_eq_kernel(long long neq, const long long* __restrict__ s
 
 // receiver traces: row[r] = x2[slot[r]]
 __global__ void record_kernel(int nrec, const long long* __restrict__ slot, const double* __restrict__ x2,
-                              double* __restrict__ row) {
+                              double* __restrict__ row, const int* __restrict__ step, long long stride) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (step) row += (long long)(*step) * stride;
   if (r < nrec) row[r] = x2[slot[r]];
 }
 
-// displacement of the regular-domain node box -> history slice (3 x nbox)
+// device-side step index of a captured time loop
+__global__ void step_set_kernel(int* __restrict__ step, int value) {
+  if (threadIdx.x == 0) step[0] = value;
+}
+__global__ void step_advance_kernel(int* __restrict__ step, int delta) {
+  if (threadIdx.x == 0) step[0] = step[0] + delta;
+}
+
+// displacement of the regular-domain node box -> history slice (3 x nbox,
+// component-major): a thread reads one node's three contiguous components and
+// writes them to the three component planes (each plane coalesced across lanes)
 __global__ void store_box_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int bny, int bnz,
-                                 const double* __restrict__ x2, double* __restrict__ hist) {
+                                 const double* __restrict__ x2, double* __restrict__ hist,
+                                 const int* __restrict__ step, long long stride) {
   // 32-bit indices: 9 NN < 2^31 (checked by the host)
   const int nbox = bnx * bny * bnz;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 3 * nbox; t += gridDim.x * blockDim.x) {
-    const int c = t / nbox;
-    const int r = t - c * nbox;
+  if (step) hist += (long long)(*step) * stride;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nbox; r += gridDim.x * blockDim.x) {
     const int bi = r % bnx, bj = (r / bnx) % bny, bk = r / (bnx * bny);
     const int n = (bi0 + bi) + g.NX * ((bj0 + bj) + g.NY * (bk0 + bk));
-    hist[t] = x2[3 * n + c];
+    const double u0 = x2[3 * n], u1 = x2[3 * n + 1], u2 = x2[3 * n + 2];
+    hist[r] = u0;
+    hist[nbox + r] = u1;
+    hist[2 * nbox + r] = u2;
   }
 }
 
@@ -158,8 +172,10 @@ constexpr int GHALO = 2 * 2 * GTX + 2 * 2 * GTY;
 __global__ void __launch_bounds__(GNT) gradient_tile_kernel(Geo g, int bi0, int bj0, int bk0, int bnx, int bny,
                                                             int bnz, const double* __restrict__ uh,
                                                             const double* __restrict__ p2, double dt,
-                                                            double* __restrict__ glam, double* __restrict__ gmu) {
+                                                            double* __restrict__ glam, double* __restrict__ gmu,
+                                                            const int* __restrict__ step, long long stride) {
   __shared__ double sU[3][GWY][GWX], sP[3][GWY][GWX];
+  if (step) uh += (long long)(*step) * stride;
   const int tid = threadIdx.x;
   const int tx = tid % GTX, ty = tid / GTX;
   const int ti0 = blockIdx.x * GTX, tj0 = blockIdx.y * GTY;
@@ -405,21 +421,30 @@ hipError_t launch_scatter_eq(long long neq, const long long* slot, const double*
   hipLaunchKernelGGL(scatter_eq_kernel, dim3(grid_for(neq)), dim3(256), 0, s, neq, slot, raw, y, sc, l, z);
   return hipGetLastError();
 }
-hipError_t launch_record(int nrec, const long long* slot, const double* x2, double* row, hipStream_t s) {
-  hipLaunchKernelGGL(record_kernel, dim3((nrec + 255) / 256), dim3(256), 0, s, nrec, slot, x2, row);
+hipError_t launch_record(int nrec, const long long* slot, const double* x2, double* row, hipStream_t s, StepRef sr) {
+  hipLaunchKernelGGL(record_kernel, dim3((nrec + 255) / 256), dim3(256), 0, s, nrec, slot, x2, row, sr.step, sr.stride);
   return hipGetLastError();
 }
-hipError_t launch_store_box(const Geo& g, const int* b0, const int* bn, const double* x2, double* hist, hipStream_t s) {
-  const long long tot = 3LL * bn[0] * bn[1] * bn[2];
+hipError_t launch_step_set(int* step, int value, hipStream_t s) {
+  hipLaunchKernelGGL(step_set_kernel, dim3(1), dim3(64), 0, s, step, value);
+  return hipGetLastError();
+}
+hipError_t launch_step_advance(int* step, int delta, hipStream_t s) {
+  hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, s, step, delta);
+  return hipGetLastError();
+}
+hipError_t launch_store_box(const Geo& g, const int* b0, const int* bn, const double* x2, double* hist, hipStream_t s,
+                            StepRef sr) {
+  const long long tot = (long long)bn[0] * bn[1] * bn[2];
   hipLaunchKernelGGL(store_box_kernel, dim3(grid_for(tot)), dim3(256), 0, s, g, b0[0], b0[1], b0[2], bn[0], bn[1],
-                     bn[2], x2, hist);
+                     bn[2], x2, hist, sr.step, sr.stride);
   return hipGetLastError();
 }
 hipError_t launch_gradient(const Geo& g, const int* b0, const int* bn, const double* uh, const double* p2, double dt,
-                           double* glam, double* gmu, hipStream_t s) {
+                           double* glam, double* gmu, hipStream_t s, StepRef sr) {
   const dim3 grid((bn[0] + GTX - 1) / GTX, (bn[1] + GTY - 1) / GTY, (bn[2] + GZC - 1) / GZC);
   hipLaunchKernelGGL(gradient_tile_kernel, grid, dim3(GNT), 0, s, g, b0[0], b0[1], b0[2], bn[0], bn[1], bn[2], uh, p2,
-                     dt, glam, gmu);
+                     dt, glam, gmu, sr.step, sr.stride);
   return hipGetLastError();
 }
 

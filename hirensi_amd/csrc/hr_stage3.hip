@@ -252,12 +252,14 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
   const int tz = b / (g.tiles0 * g.tiles1);
   const int ex0 = tx * TX, ey0 = ty * TY, ez0 = g.zb[tz];
   // this (shot, tile)'s source entries [src_j0, src_j1), read once per block (uniform)
-  int src_j0 = 0, src_j1 = 0;
+  int src_j0 = 0, src_j1 = 0, sdyn = 0;
   if (A.src_mode) {
     const __attribute__((address_space(4))) int* tptr = (const __attribute__((address_space(4))) int*)A.src.tile_ptr;
     const int tp = shot * A.src.ntiles + b;
     src_j0 = tptr[tp];
     src_j1 = tptr[tp + 1];
+    // captured time loops: the step (rows) are offsets from a device-side step index
+    if (A.step_dev) sdyn = *A.step_dev;
   }
   const int ez_end = g.zb[tz + 1];
   const int gi0 = 2 * ex0 - 2, gj0 = 2 * ey0 - 2;
@@ -801,15 +803,17 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
                   const double mm = sp->m[jj];
                   double val;
                   if (A.src_mode == 1) {
-                    const double f = sp->ftab[((size_t)shot * sp->nstep + A.step) * 3 + A.fcol];
+                    const double f = sp->ftab[((size_t)shot * sp->nstep + (A.step + sdyn)) * 3 + A.fcol];
                     val = f * (sp->val[jj] * mm);
                   } else {
                     const size_t nr = (size_t)sp->nrec;
                     const size_t base = (size_t)shot * ((size_t)sp->nstep + 1) * nr + ((size_t)jj - (size_t)shot * nr);
-                    const double va = sp->tr[base + (size_t)A.ra * nr] - sp->obs[base + (size_t)A.ra * nr];
+                    const size_t ra = (size_t)(A.ra + sdyn);
+                    const double va = sp->tr[base + ra * nr] - sp->obs[base + ra * nr];
                     val = A.ca * (va * mm);
-                    if (A.rb >= 0) {
-                      const double vb = sp->tr[base + (size_t)A.rb * nr] - sp->obs[base + (size_t)A.rb * nr];
+                    if (A.rb_on) {
+                      const size_t rb = (size_t)(A.rb + sdyn);
+                      const double vb = sp->tr[base + rb * nr] - sp->obs[base + rb * nr];
                       val = val + A.cb * (vb * mm);
                     }
                   }

@@ -161,6 +161,130 @@ module hirensi
       real(c_double), intent(out) :: ms_per_step
     end function hr_autotune
 
+    integer(c_int) function hr_get_history_plan(ctx, nstep, nshot, history, segment, seglen, nsnap, bytes, &
+        ratio) bind(C, name="hr_get_history_plan")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nstep, nshot, history, segment
+      integer(c_int), intent(out) :: seglen, nsnap
+      real(c_double), intent(out) :: bytes, ratio
+    end function hr_get_history_plan
+
+    integer(c_int) function hr_revolve_schedule(nseg, nsnap, max_actions, actions, nactions, advances) &
+        bind(C, name="hr_revolve_schedule")
+      import :: c_int, c_long_long
+      integer(c_int), value :: nseg, nsnap, max_actions
+      integer(c_int), intent(out) :: actions(*)
+      integer(c_int), intent(out) :: nactions
+      integer(c_long_long), intent(out) :: advances
+    end function hr_revolve_schedule
+
+    integer(c_int) function hr_set_graph(ctx, steps) bind(C, name="hr_set_graph")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: steps
+    end function hr_set_graph
+
+    integer(c_int) function hr_get_graph_launches(ctx, launches) bind(C, name="hr_get_graph_launches")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), intent(out) :: launches
+    end function hr_get_graph_launches
+
+    ! device pointers only (HR_DEVICE_PTRS contexts)
+    integer(c_int) function hr_misfit_noalloc(ctx, nstep, dt, nrec, dis_meas, misfit) &
+        bind(C, name="hr_misfit_noalloc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nstep, nrec
+      real(c_double), value :: dt
+      type(c_ptr), value :: dis_meas, misfit
+    end function hr_misfit_noalloc
+
+    ! learned preconditioner kernels (include/hirensi_nn.h): device pointers, a HIP stream
+    integer(c_int) function hr_nn_conv3d(x, w, bias, y, cin, cout, nb, d, h, wd, accumulate, stream) &
+        bind(C, name="hr_nn_conv3d")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: x, w, bias, y, stream
+      integer(c_int), value :: cin, cout, nb, d, h, wd, accumulate
+    end function hr_nn_conv3d
+
+    integer(c_int) function hr_nn_conv3d_wgrad(x, dy, part, nslice, dw, db, cin, cout, nb, d, h, wd, stream) &
+        bind(C, name="hr_nn_conv3d_wgrad")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: x, dy, part, dw, db, stream
+      integer(c_int), value :: nslice, cin, cout, nb, d, h, wd
+    end function hr_nn_conv3d_wgrad
+
+    integer(c_int) function hr_nn_transpose_weights(w, wt, cin, cout, stream) bind(C, name="hr_nn_transpose_weights")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: w, wt, stream
+      integer(c_int), value :: cin, cout
+    end function hr_nn_transpose_weights
+
+    integer(c_int) function hr_nn_leaky_relu(x, n, slope, stream) bind(C, name="hr_nn_leaky_relu")
+      import :: c_int, c_ptr, c_long_long, c_float
+      type(c_ptr), value :: x, stream
+      integer(c_long_long), value :: n
+      real(c_float), value :: slope
+    end function hr_nn_leaky_relu
+
+    integer(c_int) function hr_nn_leaky_relu_grad(y, dy, n, slope, stream) bind(C, name="hr_nn_leaky_relu_grad")
+      import :: c_int, c_ptr, c_long_long, c_float
+      type(c_ptr), value :: y, dy, stream
+      integer(c_long_long), value :: n
+      real(c_float), value :: slope
+    end function hr_nn_leaky_relu_grad
+
+    integer(c_int) function hr_nn_avgpool2(x, y, c, d, h, wd, stream) bind(C, name="hr_nn_avgpool2")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: x, y, stream
+      integer(c_long_long), value :: c
+      integer(c_int), value :: d, h, wd
+    end function hr_nn_avgpool2
+
+    integer(c_int) function hr_nn_avgpool2_grad(dy, dx, c, d, h, wd, accumulate, stream) &
+        bind(C, name="hr_nn_avgpool2_grad")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: dy, dx, stream
+      integer(c_long_long), value :: c
+      integer(c_int), value :: d, h, wd, accumulate
+    end function hr_nn_avgpool2_grad
+
+    integer(c_int) function hr_nn_upsample2(x, y, c, d, h, wd, stream) bind(C, name="hr_nn_upsample2")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: x, y, stream
+      integer(c_long_long), value :: c
+      integer(c_int), value :: d, h, wd
+    end function hr_nn_upsample2
+
+    integer(c_int) function hr_nn_upsample2_grad(dy, dx, c, d, h, wd, stream) bind(C, name="hr_nn_upsample2_grad")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: dy, dx, stream
+      integer(c_long_long), value :: c
+      integer(c_int), value :: d, h, wd
+    end function hr_nn_upsample2_grad
+
+    integer(c_int) function hr_nn_mse(y, t, n, grad, part, loss, stream) bind(C, name="hr_nn_mse")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: y, t, grad, part, loss, stream
+      integer(c_long_long), value :: n
+    end function hr_nn_mse
+
+    integer(c_int) function hr_nn_adam(w, g, m, v, wb, n, lr, b1, b2, eps, step, stream) bind(C, name="hr_nn_adam")
+      import :: c_int, c_ptr, c_long_long, c_float
+      type(c_ptr), value :: w, g, m, v, wb, stream
+      integer(c_long_long), value :: n
+      real(c_float), value :: lr, b1, b2, eps
+      integer(c_int), value :: step
+    end function hr_nn_adam
+
+    integer(c_int) function hr_nn_to_bf16(w, wb, n, stream) bind(C, name="hr_nn_to_bf16")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: w, wb, stream
+      integer(c_long_long), value :: n
+    end function hr_nn_to_bf16
+
     real(c_double) function hr_time_function(t, lcase, fr) bind(C, name="hr_time_function")
       import :: c_double, c_int
       real(c_double), value :: t, fr

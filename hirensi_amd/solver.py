@@ -206,6 +206,16 @@ class Context:
         self._check(self.L.hr_autotune(self.h, int(nshot), int(steps), C.byref(tz), C.byref(ms)))
         return tz.value, ms.value
 
+    def set_graph(self, steps):
+        """Capture ``steps`` RK4 steps (even; 0: eager launches) of the time loops
+        into one replayed hipGraph (``hr_set_graph``)."""
+        self._check(self.L.hr_set_graph(self.h, int(steps)))
+
+    def graph_launches(self):
+        n = C.c_longlong()
+        self._check(self.L.hr_get_graph_launches(self.h, C.byref(n)))
+        return n.value
+
     def set_history_budget(self, nbytes):
         """HBM bytes for the HR_HISTORY_REVOLVE history (0: half the free memory)."""
         self._check(self.L.hr_set_history_budget(self.h, float(nbytes)))

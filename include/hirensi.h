@@ -94,7 +94,9 @@ int hr_reset_scaling(hr_ctx* ctx, int which);
  * x, y, z have neq entries; z may alias y. */
 int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const double* y, double* z);
 
-/* TIME_FUNCTION (IO.F90:541-733): lcase 1 sine, 2 Ricker, 3 Gaussian. */
+/* TIME_FUNCTION (IO.F90:541-733), every LCASE: 1 sine, 2 Ricker, 3 Gaussian,
+ * 4 Chopra cosine, 5 its derivative, 6 Greenberg delta, 7 Ricker derivative,
+ * 8 one, 9 zero, 10 smooth Heaviside; NaN for other cases. */
 double hr_time_function(double t, int lcase, double fr);
 
 /* Forward RK4 from rest (EXPLICIT_RK4_PETSC_PML_3D).
@@ -271,6 +273,17 @@ int hr_get_kernel_stats(hr_ctx* ctx, long long* launches, double* ms);
  * and its time per step (ms).  Invalidates the history of the last forward.
  * No counterpart in the reference (a PETSc AIJ product has no tiling). */
 int hr_autotune(hr_ctx* ctx, int nshot, int steps, int* tz_best, double* ms_per_step);
+
+/* Captured time loops: with steps > 0 (even), the time loops of hr_forward /
+ * hr_forward_shots (history NONE or STORE, no energy diagnostic) and of
+ * hr_adjoint_gradient (STORE history) capture `steps` RK4 steps into one
+ * hipGraph and replay it; the per-step values come from a device-side step
+ * index, and results are bit-identical to the eager launches.  0 (the default):
+ * eager launches.  Checkpointed and revolve histories and profiling runs stay
+ * eager.  No counterpart in the reference (PETSc calls MatMultAdd per step). */
+int hr_set_graph(hr_ctx* ctx, int steps);
+/* hipGraph replays since the context was created. */
+int hr_get_graph_launches(hr_ctx* ctx, long long* launches);
 
 #ifdef __cplusplus
 }

@@ -116,6 +116,13 @@ def test_cabi_library_exports_every_declared_symbol():
     assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["hirensi.h", "hirensi_nn.h"]
 
 
+def test_fortran_module_binds_every_declared_symbol():
+    src = open(os.path.join(ROOT, "hirensi_amd", "fortran", "hirensi_mod.F90")).read()
+    bound = set(re.findall(r'bind\(C, name="(hr_\w+)"\)', src))
+    declared = set(_header_functions()) | set(_header_functions("hirensi_nn.h"))
+    assert declared <= bound, sorted(declared - bound)
+
+
 def test_host_time_function_matches_oracle():
     from hirensi_amd import solver as S
     from oracle import oracle as O
