@@ -313,7 +313,7 @@ def fwi_bench(a):
     ctx = S.Context(mesh)
     ctx.set_material(cfg["true"])
     _log(f"fwi {a.fwi}: neq {mesh.neq}, {len(cfg['src_nodes'])} shots, {nstep} steps; observing the true model")
-    obs = F.HipProblem(ctx, sv, cfg["true"].rho).observe(cfg["true"].lam, cfg["true"].mu)
+    obs = F.HipProblem(ctx, sv, cfg["true"].rho, log=_log).observe(cfg["true"].lam, cfg["true"].mu)
     sv.dis_meas = obs
     _, restriction = S.gradient_mass(ctx)
     start, true = cfg["start"], cfg["true"]
@@ -321,10 +321,10 @@ def fwi_bench(a):
     variants = ["plain", "batched"] if a.fwi_variant == "both" else [a.fwi_variant]
     for v in variants:
         if v == "plain":
-            prob = F.SequentialHipProblem(ctx, sv, start.rho, params)
+            prob = F.SequentialHipProblem(ctx, sv, start.rho, params, log=_log)
         else:
             ctx.set_history_budget(0.0)
-            prob = F.HipProblem(ctx, sv, start.rho, params, history=S.HR_HISTORY_REVOLVE)
+            prob = F.HipProblem(ctx, sv, start.rho, params, history=S.HR_HISTORY_REVOLVE, log=_log)
         t0 = time.perf_counter()
         inv = F.Inversion(prob, start.lam, start.mu, restriction, params)
         recs = [{"iter": 0, "misfit": inv.misfit, "mu_error": F.model_error(mesh, inv.mu, true.mu),

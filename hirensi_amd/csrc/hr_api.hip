@@ -1059,6 +1059,12 @@ int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const doubl
   return 0;
 }
 
+int hr_synchronize(hr_ctx* ctx) {
+  if (!ctx) return 1;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   if (!ctx) return 1;
   const int serial = (version & HR_TILING_SERIAL_CLASSES) ? 1 : 0;

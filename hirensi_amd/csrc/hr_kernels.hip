@@ -305,14 +305,18 @@ __global__ void permute_cols_kernel(long long nrows, int ncols, const long long*
 // with ADD_VALUES), then the squares are summed in equation order (VecDot).
 // Group g holds the receivers gidx[gptr[g] .. gptr[g+1]), groups in ascending
 // equation order.
-__global__ void misfit_rows_kernel(int nstep, int ngrp, const int* __restrict__ gptr, const int* __restrict__ gidx,
-                                   int nrec, const double* __restrict__ tr, const double* __restrict__ ob,
-                                   double* __restrict__ part) {
-  const int nstp = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nstp > nstep) return;
+__global__ void __launch_bounds__(256) misfit_rows_kernel(int nstep, int ngrp, const int* __restrict__ gptr,
+                                                          const int* __restrict__ gidx, int nrec,
+                                                          const double* __restrict__ tr,
+                                                          const double* __restrict__ ob, double* __restrict__ part) {
+  // one wavefront per time row: lane l takes groups l, l + 64, ... (neighbouring
+  // receivers of the row, coalesced), then a fixed butterfly sums the lanes
+  const int lane = threadIdx.x & 63;
+  const int nstp = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (nstp > nstep) return;                       // whole wavefront (uniform)
   const long long row = (long long)nstp * nrec;
   double c = 0.0;
-  for (int g = 0; g < ngrp; ++g) {
+  for (int g = lane; g < ngrp; g += 64) {
     double b = 0.0;
     for (int j = gptr[g]; j < gptr[g + 1]; ++j) {
       const int r = gidx[j];
@@ -320,7 +324,8 @@ __global__ void misfit_rows_kernel(int nstep, int ngrp, const int* __restrict__ 
     }
     c += b * b;
   }
-  part[nstp] = c;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if (lane == 0) part[nstp] = c;
 }
 
 // J = sum_{n = NSTEP..1} 0.5 * part[n] * dt  (Compute_Misfit order)
@@ -487,7 +492,7 @@ hipError_t launch_sum(int n, const double* in, double* out, hipStream_t s) {
 }
 hipError_t launch_misfit(int nstep, int ngrp, const int* gptr, const int* gidx, int nrec, const double* tr,
                          const double* ob, double dt, double* part, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(misfit_rows_kernel, dim3((nstep + 1 + 127) / 128), dim3(128), 0, s, nstep, ngrp, gptr, gidx, nrec,
+  hipLaunchKernelGGL(misfit_rows_kernel, dim3((nstep + 1 + 3) / 4), dim3(256), 0, s, nstep, ngrp, gptr, gidx, nrec,
                      tr, ob, part);
   hipLaunchKernelGGL(misfit_sum_kernel, dim3(1), dim3(64), 0, s, nstep, part, dt, out);
   return hipGetLastError();

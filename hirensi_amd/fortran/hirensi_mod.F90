@@ -179,6 +179,11 @@ module hirensi
       integer(c_long_long), intent(out) :: advances
     end function hr_revolve_schedule
 
+    integer(c_int) function hr_synchronize(ctx) bind(C, name="hr_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function hr_synchronize
+
     integer(c_int) function hr_set_graph(ctx, steps) bind(C, name="hr_set_graph")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

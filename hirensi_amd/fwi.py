@@ -24,6 +24,7 @@ The forward and gradient work is behind a *problem* object
 from __future__ import annotations
 
 import math
+import time
 from collections import deque
 from dataclasses import dataclass
 
@@ -284,23 +285,31 @@ class HipProblem:
     regularisation of Program_Main.F90:396-481."""
 
     def __init__(self, ctx: S.Context, survey: Survey, rho, params: InversionParams | None = None, wreg=None,
-                 history=S.HR_HISTORY_STORE, segment=64):
+                 history=S.HR_HISTORY_STORE, segment=64, log=None):
         self.ctx, self.sv, self.rho = ctx, survey, rho
         self.P = params or InversionParams()
         self.wreg = wreg
         self.history, self.segment = history, segment
         self.src = ctx.mesh.material_map()
         self.nforward = 0
+        self.log = log                   # log(message) after each forward and gradient, or None
+
+    def _done(self, what, t0):
+        if self.log is not None:
+            self.log(f"{what} done ({time.perf_counter() - t0:.1f} s)")
 
     def extend(self, v):
         return v[self.src]
 
     def _forward(self, lam, mu, history):
         sv = self.sv
+        t0 = time.perf_counter()
         self.ctx.set_material(Model(lam, mu, self.rho))
         self.nforward += 1
-        return S.explicit_rk4_petsc_pml_3d_shots(self.ctx, sv.nstep, sv.dt, sv.ftab, sv.src_shot, sv.src_eq,
-                                                 sv.src_val, sv.rec_eq, history=history, segment=self.segment)
+        tr = S.explicit_rk4_petsc_pml_3d_shots(self.ctx, sv.nstep, sv.dt, sv.ftab, sv.src_shot, sv.src_eq,
+                                               sv.src_val, sv.rec_eq, history=history, segment=self.segment)
+        self._done(f"forward {self.nforward}", t0)
+        return tr
 
     def observe(self, lam, mu):
         """Synthetic observations of a model (traces of every shot)."""
@@ -313,8 +322,10 @@ class HipProblem:
 
     def gradient(self, it, lam, mu):
         sv = self.sv
+        t0 = time.perf_counter()
         g_l, g_m, _ = S.explicit_rk4_petsc_pml_3d_adjoint_otd(self.ctx, sv.nstep, sv.dt, sv.rec_eq, sv.dis_meas,
                                                                with_misfit=False)
+        self._done("adjoint gradient", t0)
         c_l, c_m, _, _ = S.regularize_gradient(self.ctx, it, g_l, g_m, lam, mu, self.P.regularization,
                                                self.P.regularization_factor, self.wreg)
         return g_l, g_m, c_l, c_m
@@ -326,15 +337,19 @@ class SequentialHipProblem(HipProblem):
     whole when it fits the HBM budget, else checkpointed (HR_HISTORY_REVOLVE).
     Misfit and gradient are summed over the shots."""
 
-    def __init__(self, ctx: S.Context, survey: Survey, rho, params: InversionParams | None = None, wreg=None):
-        super().__init__(ctx, survey, rho, params, wreg, history=S.HR_HISTORY_REVOLVE)
+    def __init__(self, ctx: S.Context, survey: Survey, rho, params: InversionParams | None = None, wreg=None,
+                 log=None):
+        super().__init__(ctx, survey, rho, params, wreg, history=S.HR_HISTORY_REVOLVE, log=log)
 
     def _shot(self, q, history):
         sv = self.sv
         sel = sv.src_shot == q
+        t0 = time.perf_counter()
         self.nforward += 1
-        return S.explicit_rk4_petsc_pml_3d(self.ctx, sv.nstep, sv.dt, sv.ftab[q], sv.src_eq[sel], sv.src_val[sel],
-                                           sv.rec_eq, history=history)
+        tr = S.explicit_rk4_petsc_pml_3d(self.ctx, sv.nstep, sv.dt, sv.ftab[q], sv.src_eq[sel], sv.src_val[sel],
+                                         sv.rec_eq, history=history)
+        self._done(f"forward {self.nforward} (shot {q})", t0)
+        return tr
 
     def observe(self, lam, mu):
         self.ctx.set_material(Model(lam, mu, self.rho))
@@ -355,8 +370,10 @@ class SequentialHipProblem(HipProblem):
         g_m = np.zeros(self.ctx.nnode)
         for q in range(sv.ftab.shape[0]):
             self._shot(q, self.history)
+            t0 = time.perf_counter()
             gl, gm, _ = S.explicit_rk4_petsc_pml_3d_adjoint_otd(self.ctx, sv.nstep, sv.dt, sv.rec_eq, sv.dis_meas[q],
                                                                 with_misfit=False)
+            self._done(f"adjoint gradient (shot {q})", t0)
             g_l += gl
             g_m += gm
         c_l, c_m, _, _ = S.regularize_gradient(self.ctx, it, g_l, g_m, lam, mu, self.P.regularization,

@@ -24,7 +24,7 @@ EXPORTS = (
     "hr_forward_shots", "hr_gradient_mass", "hr_regularization", "hr_regularize_gradient",
     "hr_forward_cd", "hr_set_history_budget", "hr_get_history_plan", "hr_revolve_schedule",
     "hr_gradient_accumulate", "hr_set_numbering", "hr_surface_force", "hr_set_energy_interval", "hr_get_energy",
-    "hr_autotune", "hr_misfit_noalloc", "hr_set_graph", "hr_get_graph_launches",
+    "hr_autotune", "hr_misfit_noalloc", "hr_set_graph", "hr_get_graph_launches", "hr_synchronize",
 )
 
 # include/hirensi_nn.h (signatures set by hirensi_amd.precond.lib)
@@ -117,6 +117,7 @@ def load():
     L.hr_get_energy.argtypes = [_vp, _dp, C.POINTER(C.c_int)]
     L.hr_misfit_noalloc.argtypes = [_vp, C.c_int, C.c_double, C.c_int, _vp, _vp]
     L.hr_set_graph.argtypes = [_vp, C.c_int]
+    L.hr_synchronize.argtypes = [_vp]
     L.hr_get_graph_launches.argtypes = [_vp, C.POINTER(C.c_longlong)]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int

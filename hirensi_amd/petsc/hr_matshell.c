@@ -9,7 +9,8 @@
  *   MatMultAdd, MatMultTransposeAdd     (Solver_PETSc_Explicit.F90:1005-1034, 1323-1352)
  *   MatZeroEntries, MatAssemblyBegin/End (Program_Main.F90:692-697, Solve_Forward.F90:63-85)
  * which is exactly the operation set this shell provides.  Vectors keep the
- * reference's numbering; hr_apply reads and writes them in place.
+ * reference's numbering; hr_apply reads and writes them in place, in host
+ * memory or, for VECHIP vectors and a device-pointer context, in HBM.
  *
  * Builds only where PETSc is installed (PETSC_DIR); single process (one
  * rank owns all equations, as the one-GPU context does).
@@ -20,7 +21,29 @@
 typedef struct {
   hr_ctx* ctx;
   int which;
+  int device;   /* the context takes device pointers (HR_DEVICE_PTRS): vectors stay in HBM */
 } HrShell;
+
+/* Array of a vector in the memory the context takes: with `device`, the
+ * vector must live on the GPU (VECHIP) and its device array is passed without
+ * a copy; otherwise the host array (the library stages it). */
+static PetscErrorCode HrGetRead(const HrShell* s, Vec v, const PetscScalar** a) {
+  PetscMemType mt;
+  PetscErrorCode ierr;
+  ierr = VecGetArrayReadAndMemType(v, a, &mt); CHKERRQ(ierr);
+  if (PetscMemTypeDevice(mt) != (s->device != 0))
+    SETERRQ(PETSC_COMM_SELF, PETSC_ERR_ARG_WRONG, "hirensi: vector memory does not match the context");
+  return 0;
+}
+
+static PetscErrorCode HrGet(const HrShell* s, Vec v, PetscScalar** a) {
+  PetscMemType mt;
+  PetscErrorCode ierr;
+  ierr = VecGetArrayAndMemType(v, a, &mt); CHKERRQ(ierr);
+  if (PetscMemTypeDevice(mt) != (s->device != 0))
+    SETERRQ(PETSC_COMM_SELF, PETSC_ERR_ARG_WRONG, "hirensi: vector memory does not match the context");
+  return 0;
+}
 
 #define HRCHK(ctx, call)                                                               \
   do {                                                                                 \
@@ -34,18 +57,21 @@ static PetscErrorCode HrMultAddImpl(Mat A, Vec x, Vec y, Vec z, int transpose) {
   PetscScalar* za;
   PetscErrorCode ierr;
   ierr = MatShellGetContext(A, (void**)&s); CHKERRQ(ierr);
-  ierr = VecGetArrayRead(x, &xa); CHKERRQ(ierr);
+  ierr = HrGetRead(s, x, &xa); CHKERRQ(ierr);
   if (y == z) {
-    ierr = VecGetArray(z, &za); CHKERRQ(ierr);
+    ierr = HrGet(s, z, &za); CHKERRQ(ierr);
     HRCHK(s->ctx, hr_apply(s->ctx, s->which, transpose, xa, za, za));
   } else {
-    ierr = VecGetArrayRead(y, &ya); CHKERRQ(ierr);
-    ierr = VecGetArray(z, &za); CHKERRQ(ierr);
+    ierr = HrGetRead(s, y, &ya); CHKERRQ(ierr);
+    ierr = HrGet(s, z, &za); CHKERRQ(ierr);
     HRCHK(s->ctx, hr_apply(s->ctx, s->which, transpose, xa, ya, za));
-    ierr = VecRestoreArrayRead(y, &ya); CHKERRQ(ierr);
   }
-  ierr = VecRestoreArray(z, &za); CHKERRQ(ierr);
-  ierr = VecRestoreArrayRead(x, &xa); CHKERRQ(ierr);
+  /* device arrays: the product is queued on the library's stream; PETSc may
+   * use z on its own stream next */
+  if (s->device) HRCHK(s->ctx, hr_synchronize(s->ctx));
+  if (y != z) { ierr = VecRestoreArrayReadAndMemType(y, &ya); CHKERRQ(ierr); }
+  ierr = VecRestoreArrayAndMemType(z, &za); CHKERRQ(ierr);
+  ierr = VecRestoreArrayReadAndMemType(x, &xa); CHKERRQ(ierr);
   return 0;
 }
 
@@ -69,11 +95,12 @@ static PetscErrorCode HrDiagonalScale(Mat A, Vec l, Vec r) {
   const PetscScalar *la = NULL, *ra = NULL;
   PetscErrorCode ierr;
   ierr = MatShellGetContext(A, (void**)&s); CHKERRQ(ierr);
-  if (l) { ierr = VecGetArrayRead(l, &la); CHKERRQ(ierr); }
-  if (r) { ierr = VecGetArrayRead(r, &ra); CHKERRQ(ierr); }
+  if (l) { ierr = HrGetRead(s, l, &la); CHKERRQ(ierr); }
+  if (r) { ierr = HrGetRead(s, r, &ra); CHKERRQ(ierr); }
   HRCHK(s->ctx, hr_diagonal_scale(s->ctx, s->which, la, ra));
-  if (l) { ierr = VecRestoreArrayRead(l, &la); CHKERRQ(ierr); }
-  if (r) { ierr = VecRestoreArrayRead(r, &ra); CHKERRQ(ierr); }
+  if (s->device) HRCHK(s->ctx, hr_synchronize(s->ctx));
+  if (l) { ierr = VecRestoreArrayReadAndMemType(l, &la); CHKERRQ(ierr); }
+  if (r) { ierr = VecRestoreArrayReadAndMemType(r, &ra); CHKERRQ(ierr); }
   return 0;
 }
 
@@ -101,8 +128,11 @@ static PetscErrorCode HrDestroy(Mat A) {
   return 0;
 }
 
-/* Create the shell for operator `which` (HR_K, HR_C, HR_G) of `ctx`. */
-PetscErrorCode HrMatCreateShell(MPI_Comm comm, hr_ctx* ctx, int which, Mat* A) {
+/* Create the shell for operator `which` (HR_K, HR_C, HR_G) of `ctx`.
+ * device: ctx was created with HR_DEVICE_PTRS and the driver's vectors are
+ * VECHIP (their device arrays go to the GPU operator without a copy); 0: host
+ * vectors. */
+PetscErrorCode HrMatCreateShellMem(MPI_Comm comm, hr_ctx* ctx, int which, int device, Mat* A) {
   long long neq, nnode, nstore;
   HrShell* s;
   PetscErrorCode ierr;
@@ -110,6 +140,7 @@ PetscErrorCode HrMatCreateShell(MPI_Comm comm, hr_ctx* ctx, int which, Mat* A) {
   ierr = PetscNew(&s); CHKERRQ(ierr);
   s->ctx = ctx;
   s->which = which;
+  s->device = device;
   ierr = MatCreateShell(comm, (PetscInt)neq, (PetscInt)neq, (PetscInt)neq, (PetscInt)neq, s, A); CHKERRQ(ierr);
   ierr = MatShellSetOperation(*A, MATOP_MULT, (void (*)(void))HrMult); CHKERRQ(ierr);
   ierr = MatShellSetOperation(*A, MATOP_MULT_ADD, (void (*)(void))HrMultAdd); CHKERRQ(ierr);
@@ -122,7 +153,16 @@ PetscErrorCode HrMatCreateShell(MPI_Comm comm, hr_ctx* ctx, int which, Mat* A) {
   return 0;
 }
 
-/* Fortran binding: CALL HrMatCreateShellF(ctx, which, A, ierr) */
+/* Host vectors (the reference's VECMPI/VECSEQ on one rank). */
+PetscErrorCode HrMatCreateShell(MPI_Comm comm, hr_ctx* ctx, int which, Mat* A) {
+  return HrMatCreateShellMem(comm, ctx, which, 0, A);
+}
+
+/* Fortran binding: CALL HrMatCreateShellF(ctx, which, A, ierr) (host vectors),
+ * CALL HrMatCreateShellDevF(ctx, which, A, ierr) (VECHIP vectors). */
 void hrmatcreateshellf_(hr_ctx** ctx, int* which, Mat* A, PetscErrorCode* ierr) {
-  *ierr = HrMatCreateShell(PETSC_COMM_WORLD, *ctx, *which, A);
+  *ierr = HrMatCreateShellMem(PETSC_COMM_WORLD, *ctx, *which, 0, A);
+}
+void hrmatcreateshelldevf_(hr_ctx** ctx, int* which, Mat* A, PetscErrorCode* ierr) {
+  *ierr = HrMatCreateShellMem(PETSC_COMM_WORLD, *ctx, *which, 1, A);
 }

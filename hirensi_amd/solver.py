@@ -206,6 +206,10 @@ class Context:
         self._check(self.L.hr_autotune(self.h, int(nshot), int(steps), C.byref(tz), C.byref(ms)))
         return tz.value, ms.value
 
+    def synchronize(self):
+        """Wait for the work queued on the context's stream (``hr_synchronize``)."""
+        self._check(self.L.hr_synchronize(self.h))
+
     def set_graph(self, steps):
         """Capture ``steps`` RK4 steps (even; 0: eager launches) of the time loops
         into one replayed hipGraph (``hr_set_graph``)."""

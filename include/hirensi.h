@@ -91,8 +91,15 @@ int hr_diagonal_scale(hr_ctx* ctx, int which, const double* l, const double* r);
 int hr_reset_scaling(hr_ctx* ctx, int which);
 
 /* z = y + A_eff x (MatMultAdd) or z = y + A_eff^T x (MatMultTransposeAdd).
- * x, y, z have neq entries; z may alias y. */
+ * x, y, z have neq entries; z may alias y.  Host vectors: returns with z
+ * written.  HR_DEVICE_PTRS: returns with the product queued on the context's
+ * stream (hr_synchronize waits for it). */
 int hr_apply(hr_ctx* ctx, int which, int transpose, const double* x, const double* y, double* z);
+
+/* Waits until the work the context has queued is done (device-pointer
+ * contexts: hr_apply, hr_diagonal_scale).  No counterpart in the reference
+ * (PETSc's host MatMultAdd returns with its result). */
+int hr_synchronize(hr_ctx* ctx);
 
 /* TIME_FUNCTION (IO.F90:541-733), every LCASE: 1 sine, 2 Ricker, 3 Gaussian,
  * 4 Chopra cosine, 5 its derivative, 6 Greenberg delta, 7 Ricker derivative,
