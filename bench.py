@@ -63,6 +63,11 @@ def parse():
     p.add_argument("--pc-seed", type=int, default=5, help="seed of the inversion's true model")
     p.add_argument("--pc-weights", default="", help="load trained weights (.npz) instead of training")
     p.add_argument("--pc-save", default="", help="save the trained weights (.npz) here")
+    p.add_argument("--pc-variants", default="plain,learned",
+                   help="inversions to compare: plain (batched shots, stored history), plain-seq (one shot per "
+                        "solve, revolve history), learned (batched, stored, learned H0), learned-revolve "
+                        "(batched, revolve history, learned H0); the last learned one is compared with the "
+                        "first plain one")
     p.add_argument("--fd-check", action="store_true",
                    help="instead of timing: check the lambda/mu gradient against central finite differences "
                         "along 5 seeded smooth directions on this mesh (use with --nstep)")
@@ -171,6 +176,7 @@ def fd_check(a):
     from hirensi_amd import solver as S
     from hirensi_amd.mesh import config2, receiver_dofs
     from tests.test_gpu_fd_gradient import smooth_directions
+    _log("fd-check: building the 64x64x32 mesh and models")
     true_cfg, start_cfg = config2(with_anomaly=True), config2(with_anomaly=False)
     mesh = true_cfg["mesh"]
     nstep, dt, fr = a.nstep, true_cfg["dt"], true_cfg["fr"]
@@ -179,9 +185,11 @@ def fd_check(a):
     one = np.array([1.0])
     ftab = S.time_table(nstep, dt, 2, fr)
     ctx = S.Context(mesh, true_cfg["model"])
+    _log("fd-check: context ready; observed data")
     dobs = S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, ftab, src_eq, one, rec_eq,
                                        history=S.HR_HISTORY_NONE).copy()
     start = start_cfg["model"]
+    _log("fd-check: observed data done; gradient")
 
     def misfit(model):
         ctx.set_material(model)
@@ -192,6 +200,7 @@ def fd_check(a):
     S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, ftab, src_eq, one, rec_eq, history=S.HR_HISTORY_CHECKPOINT,
                                 segment=64)
     gl, gm, j0 = S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dobs)
+    _log("fd-check: gradient done; central differences")
     rows = []
     for t, d in enumerate(smooth_directions(mesh, 5)):
         for field, g in (("lambda", gl), ("mu", gm)):
@@ -364,6 +373,11 @@ def fwi_bench(a):
     print(json.dumps(out), flush=True)
 
 
+# variant -> (batched shots, history, learned H0)
+PC_VARIANTS = {"plain": (True, 1, False), "plain-seq": (False, 3, False), "learned": (True, 1, True),
+               "learned-revolve": (True, 3, True)}
+
+
 def precond_bench(a):
     """Learned gradient preconditioner: training pairs from the FP64 solver on
     random layered models with Gaussian inclusions (32x32x16 elements, PML 4, 4
@@ -416,9 +430,18 @@ def precond_bench(a):
     params = F.InversionParams(m_lbfgs=5, bound_lambda_lower=40.0, bound_lambda_upper=1200.0, bound_mu_lower=20.0,
                                bound_mu_upper=600.0, fac_max_update_lambda=10.0, fac_max_update_mu=10.0)
     runs = {}
-    for v in ("plain", "learned"):
-        prob = F.HipProblem(ctx, sv, start.rho, params)
-        h0 = PC.LearnedH0(net, mesh, restriction) if v == "learned" else None
+    ctx.set_history_budget(a.ckpt_budget)     # the revolve variants' HBM budget (0: half the free memory)
+    variants = [v.strip() for v in a.pc_variants.split(",") if v.strip()]
+    bad = [v for v in variants if v not in PC_VARIANTS]
+    if bad or not any(v.startswith("plain") for v in variants) or not any(v.startswith("learned") for v in variants):
+        raise SystemExit(f"--pc-variants: one plain and one learned variant of {sorted(PC_VARIANTS)}")
+    for v in variants:
+        batched, history, learned = PC_VARIANTS[v]
+        if batched:
+            prob = F.HipProblem(ctx, sv, start.rho, params, history=history, log=_log)
+        else:
+            prob = F.SequentialHipProblem(ctx, sv, start.rho, params, log=_log)
+        h0 = PC.LearnedH0(net, mesh, restriction) if learned else None
         t0 = time.perf_counter()
         inv = F.Inversion(prob, start.lam, start.mu, restriction, params, h0_map=h0)
         recs = [{"iter": 0, "misfit": inv.misfit, "mu_error": F.model_error(mesh, inv.mu, true.mu),
@@ -435,21 +458,31 @@ def precond_bench(a):
                          "fallback": r.get("fallback"), "wall_s": time.perf_counter() - t0})
             _log(f"precond {v}: iteration {r['iter']} misfit {r['misfit']:.6e} mu error {recs[-1]['mu_error']:.4f}"
                  f"{' (learned step)' if r.get('learned_h0') else ''} ({recs[-1]['wall_s']:.1f} s)")
-        runs[v] = {"records": recs, "forward_solves": prob.nforward, "stopped": stop}
+        runs[v] = {"records": recs, "forward_solves": prob.nforward, "stopped": stop,
+                   "shots": "batched" if batched else "one per solve",
+                   "history": {S.HR_HISTORY_STORE: "store", S.HR_HISTORY_REVOLVE: "revolve"}[history],
+                   "recompute_ratio": ctx.recompute_ratio()}
         if h0 is not None:
             runs[v]["h0_calls"], runs[v]["h0_used"] = h0.calls, h0.used
     ctx.close()
-    target = runs["plain"]["records"][-1]["misfit"]
-    hit = [r for r in runs["learned"]["records"] if r["misfit"] <= target]
+    plain = next(v for v in variants if v.startswith("plain"))
+    learned = [v for v in variants if v.startswith("learned")][-1]
+    target = runs[plain]["records"][-1]["misfit"]
+    hit = [r for r in runs[learned]["records"] if r["misfit"] <= target]
     out["config"] = {"workload": "64x64x32 el, PML 6, 4 shots at 30 Hz, 400 receivers", "nstep": a.pc_fwi_nstep,
-                     "iterations": a.pc_fwi_iters, "true_model_seed": a.pc_seed}
+                     "iterations": a.pc_fwi_iters, "true_model_seed": a.pc_seed, "plain": plain, "learned": learned}
     out["runs"] = runs
     out["plain_final_misfit"] = target
-    out["plain_time_s"] = runs["plain"]["records"][-1]["wall_s"]
+    out["plain_time_s"] = runs[plain]["records"][-1]["wall_s"]
     out["value"] = hit[0]["wall_s"] if hit else None
     out["learned_iterations_to_plain_misfit"] = hit[0]["iter"] if hit else None
     if hit:
         out["speedup"] = out["plain_time_s"] / hit[0]["wall_s"]
+    # the same comparison on the model error (relative L2 to the true mu)
+    target_e = runs[plain]["records"][-1]["mu_error"]
+    hit_e = [r for r in runs[learned]["records"] if r["mu_error"] <= target_e]
+    out["plain_final_mu_error"] = target_e
+    out["learned_time_to_plain_mu_error_s"] = hit_e[0]["wall_s"] if hit_e else None
     print(json.dumps(out), flush=True)
 
 

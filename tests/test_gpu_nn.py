@@ -308,3 +308,45 @@ def test_training_reduces_the_loss():
     net2 = PC.ConvNet(seed=2)
     net2.load_state(st)
     assert np.array_equal(net.infer(X[0]), net2.infer(X[0]))
+
+
+def test_solver_made_pairs_training_and_learned_h0_in_an_inversion():
+    """The whole learned-preconditioner path on a small mesh: training pairs from
+    the FP64 solver (first-iteration gradient of a layered start model, true mu
+    error), a few epochs, the H0 map's normalisation, and inversion iterations
+    that use it."""
+    from hirensi_amd import fwi as F
+    from hirensi_amd import precond as PC
+    from hirensi_amd import solver as S
+    from hirensi_amd.mesh import BoxMesh
+    mesh = BoxMesh(16, 16, 8, npml=(2,) * 5)
+    print(f"pairs + training + inversion on {mesh.nex}x{mesh.ney}x{mesh.nez}, neq={mesh.neq}", flush=True)
+    X, T = PC.generate_pairs(4, seed=3, mesh=mesh, nstep=150)
+    mask = PC.rd_voxel_mask(mesh)
+    assert X.shape == (4, mesh.nez, mesh.ney, mesh.nex) and np.isfinite(X).all() and np.isfinite(T).all()
+    for k in range(4):
+        assert abs(float(np.sqrt((X[k].astype(np.float64) ** 2).sum() / mask.sum())) - 1.0) < 1e-5
+        assert np.all(X[k][mask == 0] == 0) and np.all(T[k][mask == 0] == 0)
+    net = PC.ConvNet(seed=4)
+    hist = PC.train(net, X, T, epochs=3, batch=2, lr=1e-3)
+    assert np.isfinite(hist).all()
+    # an inversion on a case of the same family, with the map as H0
+    rng = np.random.default_rng(7)
+    true, start = PC.random_case(mesh, rng)
+    sv = PC.training_survey(mesh, 150)
+    ctx = S.Context(mesh)
+    sv.dis_meas = F.HipProblem(ctx, sv, true.rho).observe(true.lam, true.mu)
+    _, restriction = S.gradient_mass(ctx)
+    h0 = PC.LearnedH0(net, mesh, restriction)
+    q = np.random.default_rng(1).standard_normal(mesh.nnode) * restriction
+    r = h0(q)
+    if r is not None:
+        assert abs(float(r @ q) - float(q @ q)) < 1e-9 * float(q @ q)
+    params = F.InversionParams(m_lbfgs=3, bound_lambda_lower=10.0, bound_lambda_upper=2000.0, bound_mu_lower=5.0,
+                               bound_mu_upper=1000.0, fac_max_update_lambda=5.0, fac_max_update_mu=5.0)
+    inv = F.Inversion(F.HipProblem(ctx, sv, start.rho, params), start.lam, start.mu, restriction, params, h0_map=h0)
+    recs = [inv.iterate() for _ in range(2)]
+    for rec in recs:
+        assert rec["misfit"] <= rec["misfit_pre"] and ("learned_h0" in rec and "fallback" in rec)
+    assert h0.calls >= 3
+    ctx.close()
