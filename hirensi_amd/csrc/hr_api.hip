@@ -14,6 +14,7 @@
 #include "hr_device.h"
 #include "hr_revolve.h"
 #include "hr_internal.h"
+#include "hr_split.h"
 #include "../../include/hirensi.h"
 
 #pragma clang fp contract(off)
@@ -24,7 +25,8 @@ using namespace hr;
 namespace {
 constexpr int DEFAULT_TZ = 7;   // element layers per z-chunk (sweep of 3..8 at 64x64x32: 7 first)
 constexpr int DEFAULT_TY = 3;
-constexpr int DEFAULT_VERSION = 3;
+constexpr int DEFAULT_VERSION = 3;   // 3 = fused tile classes, 4 = split stage path
+constexpr int SPLIT_MAX_AXIS = 1024;  // lattice nodes per axis of the split path (packed node indices)
 
 struct HipError {
   hipError_t e;
@@ -145,6 +147,19 @@ struct hr_ctx {
   // main stream after the next stage launch's fork, so that it runs beside the
   // side classes instead of between stages.  The next launch_stage consumes it.
   std::vector<std::function<int()>> deferred;
+  // split stage path (version 4): PML nodes in lattice order (packed i | j << 10 |
+  // k << 20, and their lattice indices on the host), the tile list of its
+  // regular-domain kernel (the mixed and regular-domain tiles), the flux records
+  // and interface sums of split_shots shots, source entries by pull block
+  std::vector<std::pair<long long, int>> pnode_pos;   // (lattice index, list position), by lattice index
+  unsigned* d_pnodes = nullptr;
+  int pnblk = 0, pnblk_bulk = 0;              // blocks of the list; the first pnblk_bulk hold non-interface nodes
+  int* d_tl_split = nullptr;
+  int ntl_split = 0;
+  double* d_D = nullptr;
+  double* d_E = nullptr;
+  int split_shots = 0;
+  int *d_psrc_ptr = nullptr, *d_psrc_idx = nullptr, *d_prec_ptr = nullptr, *d_prec_idx = nullptr;
   // stage-kernel version and tile-class lists
   int version = DEFAULT_VERSION;
   std::vector<int> zb;                      // z-chunk boundaries (element layers)
@@ -286,6 +301,38 @@ int tile_of(const hr_ctx* ctx, int i, int j, int k) {
 
 int ntiles(const Geo& g) { return g.tiles[0] * g.tiles[1] * g.tiles[2]; }
 
+template <class T>
+int upload(hr_ctx* ctx, T** d, const std::vector<T>& h) {
+  if (int rc = dalloc(ctx, d, h.size())) return rc;
+  if (!h.empty()) HIPC(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Source entries at PML nodes by block of the split path's nodal pull: the
+// (shot, block) lists of entry indices, in entry order (ptr: the (shot, tile)
+// lists of nshot shots over nt tiles)
+int build_pull_blocks(hr_ctx* ctx, const std::vector<int>& ptr, const std::vector<long long>& node, int nshot, int nt,
+                      int** d_ptr, int** d_idx, Src3& h3) {
+  const int nblk = ctx->pnblk;
+  std::vector<std::pair<long long, int>> kv;
+  for (int q = 0; q < nshot; ++q)
+    for (int jj = ptr[(size_t)q * nt]; jj < ptr[(size_t)(q + 1) * nt]; ++jj) {
+      auto it = std::lower_bound(ctx->pnode_pos.begin(), ctx->pnode_pos.end(), std::make_pair(node[jj], -1));
+      if (it == ctx->pnode_pos.end() || it->first != node[jj]) continue;
+      kv.push_back({(long long)q * nblk + it->second / SPLIT_NT, jj});
+    }
+  std::sort(kv.begin(), kv.end());
+  std::vector<int> bp((size_t)nshot * nblk + 1, 0), idx;
+  for (auto& e : kv) { bp[e.first + 1]++; idx.push_back(e.second); }
+  for (size_t b = 0; b + 1 < bp.size(); ++b) bp[b + 1] += bp[b];
+  if (int rc = upload(ctx, d_ptr, bp)) return rc;
+  if (int rc = upload(ctx, d_idx, idx)) return rc;
+  h3.pblk_ptr = *d_ptr;
+  h3.pblk_idx = *d_idx;
+  h3.pnblk = nblk;
+  return 0;
+}
+
 // Sparse source grouped by owning tile.  entries: (lattice node, comp, M^-1)
 struct SrcBuild {
   std::vector<int> ptr;
@@ -325,12 +372,6 @@ SrcBuild build_src(hr_ctx* ctx, int n, const long long* eq) {
   return s;
 }
 
-template <class T>
-int upload(hr_ctx* ctx, T** d, const std::vector<T>& h) {
-  if (int rc = dalloc(ctx, d, h.size())) return rc;
-  if (!h.empty()) HIPC(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-  return 0;
-}
 
 hipEvent_t take_event(hr_ctx* ctx) {
   if (!ctx->ev_pool.empty()) {
@@ -416,7 +457,8 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
   const bool counted = ctx->profiling && A.stage <= 4;
   if (counted) ctx->prof_all[A.stage] += 1;
   const bool prof = counted && (ctx->prof_tick++ % PROF_PERIOD) == 0;
-  hr_ctx::StageEv ev{A.stage, ctx->concurrent ? 3 : 1, {nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  const bool split = ctx->version == 4 && A.stage >= 1 && A.stage <= 4;
+  hr_ctx::StageEv ev{A.stage, ctx->concurrent ? (split ? 2 : 3) : 1, {nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   if (prof) {
     if (ctx->ev_open.size() >= 4096) if (int rc = drain_events(ctx)) return rc;
     for (int q = 0; q < ev.n; ++q) {
@@ -435,6 +477,43 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
     if (!((A.stage == 0 && unit_signs) || (A.stage > 0 && (A.transpose ? adj_signs : unit_signs))))
       return fail(ctx, 3, "stage sign pattern not supported by the v3 kernel");
     if (A.nsrc_tiles && !(sp && sp->mode)) return fail(ctx, 3, "v3 stage launch without its source lists");
+    if (split) {
+      // nodal pass (main stream) beside the regular-domain kernel (side stream),
+      // then the nodal pull, which reads the interface sums of the latter
+      Stage3Args B = args3(ctx, A, sp);
+      if (ctx->split_shots < B.nshot || !ctx->d_D || !ctx->d_pnodes)
+        return fail(ctx, 3, "split stage path without its buffers for %d shots", B.nshot);
+      B.pnodes = ctx->d_pnodes;
+      B.D = ctx->d_D;
+      B.E = ctx->d_E;
+      // all node blocks (the list holds the non-interface nodes, then the interface ones)
+      Stage3Args Ball = B;
+      Ball.pb0 = 0;
+      Ball.pnb = ctx->pnblk;
+      if (ctx->concurrent) {
+        HIPC(hipEventRecord(ctx->ev_fork, ctx->stream));
+        HIPC(hipStreamWaitEvent(ctx->cstream[0], ctx->ev_fork, 0));
+        if (prof) HIPC(hipEventRecord(ev.start[1], ctx->cstream[0]));
+        HIPC(launch_split_rd(B, A.stage, A.transpose, ctx->d_tl_split, ctx->ntl_split, ctx->cstream[0]));
+        if (prof) HIPC(hipEventRecord(ev.end[1], ctx->cstream[0]));
+        HIPC(hipEventRecord(ctx->ev_join[0], ctx->cstream[0]));
+        if (prof) HIPC(hipEventRecord(ev.start[0], ctx->stream));
+        HIPC(launch_split_nodal(Ball, A.stage, A.transpose, ctx->stream));
+        if (int rc = run_deferred(ctx)) return rc;
+        HIPC(hipStreamWaitEvent(ctx->stream, ctx->ev_join[0], 0));
+        HIPC(launch_split_pull(Ball, A.stage, A.transpose, ctx->stream));
+        if (prof) HIPC(hipEventRecord(ev.end[0], ctx->stream));
+      } else {
+        if (int rc = run_deferred(ctx)) return rc;
+        if (prof) HIPC(hipEventRecord(ev.start[0], ctx->stream));
+        HIPC(launch_split_nodal(Ball, A.stage, A.transpose, ctx->stream));
+        HIPC(launch_split_rd(B, A.stage, A.transpose, ctx->d_tl_split, ctx->ntl_split, ctx->stream));
+        HIPC(launch_split_pull(Ball, A.stage, A.transpose, ctx->stream));
+        if (prof) HIPC(hipEventRecord(ev.end[0], ctx->stream));
+      }
+      if (prof) ctx->ev_open.push_back(ev);
+      return 0;
+    }
     const Stage3Args B = args3(ctx, A, sp);
     if (ctx->concurrent) {
       // classes own disjoint nodes: classes 1, 2 run on the side streams
@@ -507,6 +586,13 @@ int setup_tiles(hr_ctx* ctx) {
     ctx->ntl[c] = (int)lists[c].size();
     if (int rc = upload(ctx, &ctx->d_tl[c], lists[c])) return rc;
   }
+  // the split path's regular-domain kernel runs every tile with a regular-domain element
+  {
+    std::vector<int> sl(lists[0]);
+    sl.insert(sl.end(), lists[1].begin(), lists[1].end());
+    ctx->ntl_split = (int)sl.size();
+    if (int rc = upload(ctx, &ctx->d_tl_split, sl)) return rc;
+  }
   // per-tile source lists of an earlier forward no longer match the tiles
   ctx->have_src = false;
   ctx->f_nstep = 0;
@@ -522,6 +608,19 @@ int check_ready(hr_ctx* ctx) {
 }
 
 // state buffers for nshot shots (shot-major); contents are not kept on growth
+// flux records and interface sums of the split path for shot_cap shots; the
+// records of regular-domain-only nodes are never written and stay zero (the
+// pull reads them with zero weights)
+int ensure_split(hr_ctx* ctx) {
+  if (ctx->version != 4 || ctx->split_shots >= ctx->shot_cap) return 0;
+  const size_t nd = 18 * (size_t)ctx->g.NN * ctx->shot_cap, ne = 3 * (size_t)ctx->g.NN * ctx->shot_cap;
+  if (int rc = dalloc(ctx, &ctx->d_D, nd)) return rc;
+  if (int rc = dalloc(ctx, &ctx->d_E, ne)) return rc;
+  HIPC(hipMemsetAsync(ctx->d_D, 0, nd * sizeof(double), ctx->stream));
+  ctx->split_shots = ctx->shot_cap;
+  return 0;
+}
+
 int ensure_shots(hr_ctx* ctx, int nshot) {
   const size_t nf = field_size(ctx);
   if (nshot > ctx->shot_cap) {
@@ -533,7 +632,7 @@ int ensure_shots(hr_ctx* ctx, int nshot) {
     ctx->shot_cap = nshot;
   }
   ctx->nshot = nshot;
-  return 0;
+  return ensure_split(ctx);
 }
 
 int zero_state(hr_ctx* ctx) {
@@ -788,6 +887,33 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
     HIPC(hipEventCreateWithFlags(&ctx->ev_join[q], hipEventDisableTiming));
   }
   HIPC(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+  // PML nodes of the split stage path, in lattice order; lattices wider than its
+  // packed indices use the fused tile classes
+  if (g.NX <= SPLIT_MAX_AXIS && g.NY <= SPLIT_MAX_AXIS && g.NZ <= SPLIT_MAX_AXIS && 144.0 * (double)g.NN < 4294967296.0) {
+    // non-interface PML nodes, then those a regular-domain element also holds
+    // (their pull waits for the regular-domain kernel), each part in lattice order
+    // and padded to whole blocks
+    std::vector<unsigned> pk;
+    for (int part = 0; part < 2; ++part) {
+      for (int k = 0; k < g.NZ; ++k)
+        for (int j = 0; j < g.NY; ++j)
+          for (int i = 0; i < g.NX; ++i) {
+            if (!h_pml_node(g, i, j, k)) continue;
+            const bool iface = sp_iface(sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
+                                        sp_axis(k, g.nez, g.rlo[2], g.nez));
+            if (iface != (part == 1)) continue;
+            ctx->pnode_pos.push_back({lat(g, i, j, k), (int)pk.size()});
+            pk.push_back((unsigned)i | ((unsigned)j << 10) | ((unsigned)k << 20));
+          }
+      while (pk.size() % SPLIT_NT) pk.push_back(0xffffffffu);
+      if (part == 0) ctx->pnblk_bulk = (int)(pk.size() / SPLIT_NT);
+    }
+    ctx->pnblk = (int)(pk.size() / SPLIT_NT);
+    std::sort(ctx->pnode_pos.begin(), ctx->pnode_pos.end());
+    if (int rc = upload(ctx, &ctx->d_pnodes, pk)) return rc;
+  } else {
+    ctx->version = 3;
+  }
   if (int rc = setup_tiles(ctx)) return rc;
   if (int rc = upload(ctx, &ctx->d_eq_slot, ctx->eq_slot)) return rc;
   // coordinates (origin: centred in x, y; top surface at z = 0)
@@ -814,6 +940,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
   }
   HIPC(hipMemset(ctx->tmp, 0, nf * sizeof(double)));
   HIPC(hipMemset(ctx->tmp2, 0, nf * sizeof(double)));
+  if (int rc = ensure_split(ctx)) return rc;
   if (int rc = zero_state(ctx)) return rc;
   HIPC(hipStreamSynchronize(ctx->stream));
   return 0;
@@ -836,6 +963,8 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   F(ctx->d_obs_sorted);
   F(ctx->d_img); F(ctx->d_restr); F(ctx->d_reg); F(ctx->d_cost_el); F(ctx->d_red); F(ctx->d_pm); F(ctx->d_pg);
   for (int c = 0; c < 3; ++c) F(ctx->d_tl[c]);
+  F(ctx->d_pnodes); F(ctx->d_tl_split); F(ctx->d_D); F(ctx->d_E);
+  F(ctx->d_psrc_ptr); F(ctx->d_psrc_idx); F(ctx->d_prec_ptr); F(ctx->d_prec_idx);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
@@ -1069,16 +1198,18 @@ int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   if (!ctx) return 1;
   const int serial = (version & HR_TILING_SERIAL_CLASSES) ? 1 : 0;
   version &= ~HR_TILING_SERIAL_CLASSES;
-  if (ty < 0 || tz < 0 || (version != 0 && version != 3)) return fail(ctx, 1, "bad tiling (stage kernel version 3)");
+  if (ty < 0 || tz < 0 || (version != 0 && version != 3 && version != 4))
+    return fail(ctx, 1, "bad tiling (stage kernel version 3 or 4)");
+  if (version == 4 && !ctx->d_pnodes) return fail(ctx, 1, "lattice too large for the split stage path (version 4)");
   Geo& g = ctx->g;
-  const int v = DEFAULT_VERSION;
   ctx->concurrent = serial ? 0 : 1;
   const int t = ty ? ty : DEFAULT_TY;
   if (t != 3) return fail(ctx, 1, "stage kernel tiles are 4 x 3 elements");
-  ctx->version = v;
+  if (version) ctx->version = version;
   g.tx = 4;
   g.ty = t;
   g.tz = tz ? tz : DEFAULT_TZ;
+  if (int rc = ensure_split(ctx)) return rc;
   return setup_tiles(ctx);
 }
 
@@ -1315,6 +1446,8 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
     std::memset(&h3, 0, sizeof(h3));
     h3.tile_ptr = ctx->d_src_tile_ptr; h3.node = ctx->d_src_node; h3.comp = ctx->d_src_comp;
     h3.m = ctx->d_src_m; h3.val = ctx->d_src_val; h3.ftab = ctx->d_ftab; h3.nstep = nstep; h3.ntiles = nt;
+    if (ctx->d_pnodes)
+      if (int rc = build_pull_blocks(ctx, ptr, node, nshot, nt, &ctx->d_psrc_ptr, &ctx->d_psrc_idx, h3)) return rc;
     ctx->src3_fwd = h3;
     ctx->have_src = true;
   } else {
@@ -1557,10 +1690,12 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
   // adjoint source entries: the shared receivers, grouped by tile, repeated per shot;
   // traces and observations permuted into entry order (device gather)
   SrcBuild sb = build_src(ctx, nrec, req.data());
+  std::vector<int> rptr(1, 0);
+  std::vector<long long> rnode;
   {
     const int nt = ntiles(ctx->g);
-    std::vector<int> ptr(1, 0);
-    std::vector<long long> node;
+    std::vector<int>& ptr = rptr;
+    std::vector<long long>& node = rnode;
     std::vector<int> comp;
     std::vector<double> m;
     for (int q = 0; q < nshot; ++q) {
@@ -1586,6 +1721,9 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     h3.tile_ptr = ctx->d_rec_tile_ptr; h3.node = ctx->d_rec_node; h3.comp = ctx->d_rec_comp;
     h3.m = ctx->d_rec_m; h3.tr = ctx->d_tr_sorted; h3.obs = ctx->d_obs_sorted; h3.nrec = nrec;
     h3.nstep = nstep; h3.ntiles = ntiles(ctx->g);
+    if (ctx->d_pnodes)
+      if (int rc = build_pull_blocks(ctx, rptr, rnode, nshot, h3.ntiles, &ctx->d_prec_ptr, &ctx->d_prec_idx, h3))
+        return rc;
     ctx->src3_adj = h3;
   }
   if (nrec) {

@@ -125,6 +125,11 @@ struct Src3 {                      // device-resident source entries of one solv
   int nrec;                        // adjoint: entries per shot (shared receivers)
   int nstep;
   int ntiles;
+  // the same entries at PML nodes, by block of the nodal PML pull: entries of
+  // (shot, block) are pblk_idx[pblk_ptr[s*pnblk+b] .. pblk_ptr[s*pnblk+b+1])
+  const int* pblk_ptr;
+  const int* pblk_idx;
+  int pnblk;
 };
 
 struct Stage3Args {
@@ -144,9 +149,25 @@ struct Stage3Args {
   int nshot;                       // shots in the launch; state vectors are shot-major
   long long shot_stride;           // elements per shot of a state vector (9 NN)
   double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed); stage 5: dt^2, dt, dt^3
+  // split stage path (RK4 stages): the PML node list (packed i | j << 10 | k <<
+  // 20; 0xffffffff pads), of which a nodal launch covers the SPLIT_NT-node
+  // blocks [pb0, pb0 + pnb); the flux records D (18 NN doubles per shot) and the
+  // regular-domain K sums of interface nodes E (3 NN doubles per shot)
+  const unsigned* pnodes;
+  int pb0, pnb;
+  double* D;
+  double* E;
 };
 
 // stage 0 = apply; transposed stages 1..4 carry the adjoint signs (-G, +K, -C)
 hipError_t launch_stage3(const Stage3Args& A, int stage, int transpose, int tile_class, const int* tiles, int ntiles,
                          hipStream_t s);
+// split stage path, RK4 stages 1..4: the nodal pass over the PML nodes (flux
+// records, node-local terms, stage-4 x1 x2), the regular-domain kernel over the
+// tiles with regular-domain elements (finishes regular-domain-only nodes, K
+// contribution of interface nodes into E) and the nodal pull over the PML nodes
+constexpr int SPLIT_NT = 256;      // nodes (threads) per block of the nodal kernels
+hipError_t launch_split_nodal(const Stage3Args& A, int stage, int transpose, hipStream_t s);
+hipError_t launch_split_rd(const Stage3Args& A, int stage, int transpose, const int* tiles, int ntiles, hipStream_t s);
+hipError_t launch_split_pull(const Stage3Args& A, int stage, int transpose, hipStream_t s);
 }  // namespace hr

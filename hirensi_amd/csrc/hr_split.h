@@ -1,0 +1,77 @@
+// Coefficients of the nodal PML pull (split stage path), shared by the HIP
+// kernels and the host-side checks.  Plain constexpr functions: the HIP
+// compiler treats them as callable on both sides.
+//
+// A lattice node n sits in one element per axis (odd lattice coordinate, local
+// point 1) or in up to two (even coordinate: the lower element, local point 2,
+// and the upper one, local point 0).  Every PML element e holding n adds, per
+// axis k, the transposed derivative of the nodal flux G_ik along the k-line
+// through n (u rows) and, weighted by W(n in e), the forward derivative of the
+// F-weighted input w_k along the same line (s rows).  The weights factor, W =
+// det J w(a) w(b) w(c), so along the x-line the elements on one x-side differ
+// only by w(l_y) w(l_z), a constant per axis parity: their x terms combine into
+// one stencil scaled by
+//   s_x[side] = w_y w_z * (number of PML elements among the y-z sides),
+// where regular-domain elements drop out (they contribute through sigma).  The
+// element box of the regular domain is a product of per-axis ranges, so that
+// count is (n_y n_z - r_x[side] R_y R_z) with n the sides per axis and R the
+// regular-domain sides.
+//
+// With side scales s0 (lower or only element) and s1 (upper), the u-row
+// stencil of offsets -2..2 is, from w(p) DM(p, l) of the 3-point GLL rule,
+//   even: (-s0/6, 2 s0/3, (s0 - s1)/2, -2 s1/3, s1/6),   odd: (0, 2 s0/3, 0, -2 s0/3, 0),
+// and by summation by parts (w_a DM(a, b) + w_b DM(b, a) = 0 off the end
+// points) the s-row stencil w(l) DM(l, p) is the same with the sign of every
+// offset but 0 flipped.
+#pragma once
+
+namespace hr {
+
+struct SpAxis {
+  int odd;      // odd lattice coordinate: one element
+  int v[2];     // side exists
+  int r[2];     // side exists and is inside the regular-domain element range
+};
+
+// lattice coordinate c of an axis with nel elements; regular-domain element range [lo, hi)
+constexpr SpAxis sp_axis(int c, int nel, int lo, int hi) {
+  SpAxis a{c & 1, {0, 0}, {0, 0}};
+  int e0 = 0, e1 = 0;
+  if (a.odd) {
+    a.v[0] = 1; e0 = (c - 1) >> 1;
+  } else {
+    a.v[0] = c >= 2 ? 1 : 0; e0 = (c >> 1) - 1;
+    a.v[1] = (c >> 1) < nel ? 1 : 0; e1 = c >> 1;
+  }
+  a.r[0] = (a.v[0] && e0 >= lo && e0 < hi) ? 1 : 0;
+  a.r[1] = (a.v[1] && e1 >= lo && e1 < hi) ? 1 : 0;
+  return a;
+}
+
+constexpr double sp_wn(const SpAxis& a) { return a.odd ? 4.0 / 3.0 : 1.0 / 3.0; }
+
+// side scales of axis A, B and C being the other two axes
+constexpr void sp_side_scales(const SpAxis& A, const SpAxis& B, const SpAxis& C, double& s0, double& s1) {
+  const int nb = B.v[0] + B.v[1], nc = C.v[0] + C.v[1];
+  const int rb = B.r[0] + B.r[1], rc = C.r[0] + C.r[1];
+  const double w = sp_wn(B) * sp_wn(C);
+  s0 = A.v[0] ? w * (double)(nb * nc - A.r[0] * rb * rc) : 0.0;
+  s1 = A.v[1] ? w * (double)(nb * nc - A.r[1] * rb * rc) : 0.0;
+}
+
+// u-row stencil of offsets -2..2 (index d + 2); the s-row stencil is
+// cs[2] = cu[2], cs[d] = -cu[d] otherwise
+constexpr void sp_stencil(int odd, double s0, double s1, double cu[5]) {
+  cu[0] = odd ? 0.0 : -s0 / 6.0;
+  cu[1] = 2.0 * s0 / 3.0;
+  cu[2] = odd ? 0.0 : 0.5 * (s0 - s1);
+  cu[3] = odd ? -2.0 * s0 / 3.0 : -2.0 * s1 / 3.0;
+  cu[4] = odd ? 0.0 : s1 / 6.0;
+}
+
+// whether a regular-domain element holds the node (an interface node)
+constexpr bool sp_iface(const SpAxis& X, const SpAxis& Y, const SpAxis& Z) {
+  return (X.r[0] + X.r[1]) * (Y.r[0] + Y.r[1]) * (Z.r[0] + Z.r[1]) > 0;
+}
+
+}  // namespace hr

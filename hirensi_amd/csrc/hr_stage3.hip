@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include "hr_device.h"
+#include "hr_split.h"
 
 // Fused multiply-add is allowed here: the parity bar is a tolerance, and FMA
 // halves the FP64 instruction count of the stencils.
@@ -205,8 +206,13 @@ struct Shape3 {
 // register allocations that measured 6 %, 1.5 % and 2 % faster in the whole
 // gradient (the compiler reports the unmet targets).  At stage 4 the PML-only
 // class would then need more than 256 registers.
-template <int TX, int TY, bool HAS_RD, bool HAS_PML, int S, bool TR>
+// SPLIT (regular-domain class only): the kernel also runs the tiles that hold
+// PML elements, skips them, and leaves the PML nodes to the nodal kernels: their
+// stage-4 x1, x2 updates are not written here and their regular-domain K sums go
+// to A.E instead of being finished.
+template <int TX, int TY, bool HAS_RD, bool HAS_PML, int S, bool TR, bool SPLIT = false>
 __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S <= 3 ? 3 : 2)) stage_kernel3(const Stage3Args A, const int* __restrict__ tiles, int ntl) {
+  static_assert(!SPLIT || (HAS_RD && !HAS_PML && S >= 1 && S <= 4), "split form: regular-domain RK4 stages");
   using SH = Shape3<TX, TY>;
   constexpr int WX = SH::WX, WP = SH::WP, NEW = SH::NEW, OX = SH::OX, OC = SH::OC, HP = SH::HP;
   static_assert(OC <= 64 && NT3 == 256, "one owner plane group per wavefront, plus one helper wavefront");
@@ -265,7 +271,9 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
   const int gi0 = 2 * ex0 - 2, gj0 = 2 * ey0 - 2;
   const int own_i1 = (ex0 + TX >= g.nex) ? g.NX : 2 * (ex0 + TX);
   const int own_j1 = (ey0 + TY >= g.ney) ? g.NY : 2 * (ey0 + TY);
-  const int ez_first = max(ez0 - 1, 0);
+  // split form: no regular-domain element lies below layer rlo[2], so the slabs
+  // under it hold no work for this kernel (their nodes are PML nodes)
+  const int ez_first = SPLIT ? max(max(ez0 - 1, 0), g.rlo[2]) : max(ez0 - 1, 0);
   const int mask = (S == 0) ? A.set_mask : 7;
   const bool useG = mask & 1, useK = mask & 2, useC = mask & 4;
   const unsigned NN8 = (unsigned)g.NN * 8u;   // bytes of one double per lattice node
@@ -393,7 +401,7 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
         }
       }
       if constexpr (OWN && S == 4) {
-        if (own_plane && ua) {
+        if (own_plane && ua && !(SPLIT && pml3(g, ii, jj, kk))) {
           double xn0[3], xn1[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c) rk4_update12(XU[c], A.hh, A.h, A.h6, xn0[c], xn1[c], c3o[c]);
@@ -783,6 +791,13 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
         const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
         const unsigned offu = n * 24u, offs = SB + n * 48u;
         const bool ua = u_act3(g, i, j, k);
+        if constexpr (SPLIT) {
+          if (pml3(g, i, j, k)) {
+            // interface node: its regular-domain K sums, finished by the nodal pull
+            if (ua) st3(A.E + (long long)shot * 3 * g.NN, offu, acc[0], acc[1], acc[2]);
+            goto next_slab;
+          }
+        }
         const int sm = pn ? smask3(g, i, j, k) : 0;
         double raw[9];
 #pragma unroll
@@ -856,7 +871,355 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
         }
       }
     }
+  next_slab:
     if constexpr (RING == 3) __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ split stage path
+// Blocks of SPLIT_NT consecutive entries of the PML node list (entries past the
+// nodes are 0xffffffff), one node per thread.  XCD x = blockIdx % 8 takes the
+// logical blocks [x C, (x + 1) C) of the launch, so that each XCD's L2 holds a
+// contiguous run of the list (the neighbouring rows and planes of its nodes); a
+// launch has 8 C blocks and starts at logical block A.pb0.
+__device__ __forceinline__ int split_block(int nblk) {
+  const int C = (nblk + 7) / 8;
+  return (blockIdx.x & 7) * C + (blockIdx.x >> 3);
+}
+
+__device__ __forceinline__ void st2(double* p, unsigned off, double a, double b) {
+  *reinterpret_cast<double2*>(reinterpret_cast<char*>(p) + off) = make_double2(a, b);
+}
+
+// Flux records D of the split path, per shot 18 NN doubles: three arrays (ax =
+// x, y, z) of six doubles per lattice node, (G_{x,ax}, G_{y,ax}, G_{z,ax},
+// w_{ax,x}, w_{ax,y}, w_{ax,z}) with G_{i,ax} the nodal PML flux and w_{ax,i}
+// the F-weighted inputs: a line along ax reads three 16-byte pairs per node.
+__device__ __forceinline__ unsigned d_rec(int ax, unsigned NN, unsigned n) { return 48u * ((unsigned)ax * NN + n); }
+
+// Node-local part of RK4 stage S at one PML node: its stage inputs from the raw
+// inputs (the fused kernel's PML window load, once per node), the flux records,
+// and into the stage output the node-local terms: stages 1-3 -M^-1 (diagonal
+// terms), stage 4 the carry x3 + dt/6 (g1 + 2 g2 + 2 g3) minus dt/6 M^-1
+// (diagonal terms), with the x1, x2 updates written here.
+template <int S, bool TR>
+__global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(const Stage3Args A) {
+  constexpr bool NEG = TR;
+  constexpr int NIN = 2 + S;
+  const Geo3& g = A.g;
+  const int lb = split_block(A.pnb);
+  if (lb >= A.pnb) return;
+  const int t = (A.pb0 + lb) * SPLIT_NT + (int)threadIdx.x;
+  const unsigned ijk = A.pnodes[t];
+  if (ijk == 0xffffffffu) return;
+  const int shot = blockIdx.y;
+  const int i = ijk & 1023u, j = (ijk >> 10) & 1023u, k = ijk >> 20;
+  const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
+  const long long so = (long long)shot * A.shot_stride;
+  const unsigned NN = (unsigned)g.NN;
+  const unsigned SB = 8u * (3u * NN + (NN & 1u));
+  const unsigned offu = n * 24u, offs = SB + n * 48u, offq = 16u * NN + n * 32u;
+  const double* __restrict__ tab = A.tab;
+  const double* __restrict__ nodet = A.node;
+
+  double XU[3][NIN], XS6[6][NIN];
+#pragma unroll
+  for (int q = 0; q < NIN; ++q) {
+    const double* ip = A.in[q] + so;
+    ld3(ip, offu, XU[0][q], XU[1][q], XU[2][q]);
+    ld2(ip, offs, XS6[0][q], XS6[1][q]);
+    ld2(ip, offs + 16u, XS6[2][q], XS6[3][q]);
+    ld2(ip, offs + 32u, XS6[4][q], XS6[5][q]);
+  }
+  double lam, mu, minv_u, minv_s, rwp, wps, ax, bx, ay, by, az, bz;
+  ld2(nodet, n * 16u, lam, mu);
+  ld2(nodet, offq, minv_u, minv_s);
+  ld2(nodet, offq + 16u, rwp, wps);
+  ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * i), ax, bx);
+  ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * j), ay, by);
+  ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * g.NY + 2 * k), az, bz);
+  const bool ua = u_act3(g, i, j, k);
+  const int sm = smask3(g, i, j, k);
+
+  double v1u[3], vu2[3], v3u[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) combine<S, NEG>(XU[c], A.hh, A.h, A.h6, 7, v1u[c], vu2[c], v3u[c]);
+  // F[set][k], set 0 = w (G, v1), 1 = p (K, v2), 2 = e (C, v3)
+  double F[3][3];
+  F[2][0] = ay * az; F[2][1] = ax * az; F[2][2] = ax * ay;
+  F[1][0] = ay * bz + az * by; F[1][1] = ax * bz + az * bx; F[1][2] = ax * by + ay * bx;
+  F[0][0] = by * bz; F[0][1] = bx * bz; F[0][2] = bx * by;
+  double wv[9];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) wv[3 * d + c] = ((0.0 + F[0][d] * v1u[c]) + F[1][d] * vu2[c]) + F[2][d] * v3u[c];
+  // diagonal terms rho W (b v3 + c v2 + d v1) of ELEMENT_EM/EN (PML.F90 items 7..10)
+  const double qb = ax * ay * bz + ax * az * by + ay * az * bx;
+  const double qc = ax * by * bz + ay * bx * bz + az * bx * by;
+  const double qd = bx * by * bz;
+  auto diag = [&](double wm, double a1, double a2, double a3) { return wm * (((0.0 + qb * a3) + qc * a2) + qd * a1); };
+  double dg[9], c3[9], Gv[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) { c3[m] = 0.0; Gv[m] = 0.0; }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) dg[c] = diag(rwp, v1u[c], vu2[c], v3u[c]);
+  double* const x1n = A.xnew[0] ? A.xnew[0] + so : nullptr;
+  double* const x2n = A.xnew[1] ? A.xnew[1] + so : nullptr;
+  if constexpr (S == 4) {
+    double xn0[3], xn1[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rk4_update12(XU[c], A.hh, A.h, A.h6, xn0[c], xn1[c], c3[c]);
+    if (ua) {
+      st3(x1n, offu, xn0[0], xn0[1], xn0[2]);
+      st3(x2n, offu, xn1[0], xn1[1], xn1[2]);
+    }
+  }
+  const double l2m = lam + 2.0 * mu, m2 = 2.0 * mu;
+#pragma unroll
+  for (int hf = 0; hf < 2; ++hf) {
+    const double(*XS)[NIN] = &XS6[3 * hf];
+    double s1[3], s2[3], s3[3];
+#pragma unroll
+    for (int tt = 0; tt < 3; ++tt) combine<S, NEG>(XS[tt], A.hh, A.h, A.h6, 7, s1[tt], s2[tt], s3[tt]);
+#pragma unroll
+    for (int X = 0; X < 3; ++X) {
+      const double* sv = X == 0 ? s1 : (X == 1 ? s2 : s3);
+      if (hf == 0) {
+        if constexpr (!TR) {
+          // T(s): stress of the auxiliary strain (ELEMENT_EA_3D lower block), normal part
+          const double Txx = l2m * sv[0] + lam * sv[1] + lam * sv[2];
+          const double Tyy = lam * sv[0] + l2m * sv[1] + lam * sv[2];
+          const double Tzz = lam * sv[0] + lam * sv[1] + l2m * sv[2];
+          Gv[0] -= F[X][0] * Txx; Gv[4] -= F[X][1] * Tyy; Gv[8] -= F[X][2] * Tzz;
+        } else {
+          Gv[0] += F[X][0] * sv[0]; Gv[4] += F[X][1] * sv[1]; Gv[8] += F[X][2] * sv[2];
+        }
+      } else {
+        if constexpr (!TR) {
+          const double Txy = m2 * sv[0], Txz = m2 * sv[1], Tyz = m2 * sv[2];
+          Gv[1] -= F[X][1] * Txy; Gv[2] -= F[X][2] * Txz; Gv[3] -= F[X][0] * Txy;
+          Gv[5] -= F[X][2] * Tyz; Gv[6] -= F[X][0] * Txz; Gv[7] -= F[X][1] * Tyz;
+        } else {
+          Gv[1] += F[X][1] * sv[0]; Gv[2] += F[X][2] * sv[1]; Gv[3] += F[X][0] * sv[0];
+          Gv[5] += F[X][2] * sv[2]; Gv[6] += F[X][0] * sv[1]; Gv[7] += F[X][1] * sv[2];
+        }
+      }
+    }
+    const double wm = hf == 0 ? wps : 2.0 * wps;
+#pragma unroll
+    for (int tt = 0; tt < 3; ++tt) dg[3 + 3 * hf + tt] = diag(wm, s1[tt], s2[tt], s3[tt]);
+    if constexpr (S == 4) {
+      double xn0[3], xn1[3];
+#pragma unroll
+      for (int tt = 0; tt < 3; ++tt) rk4_update12(XS[tt], A.hh, A.h, A.h6, xn0[tt], xn1[tt], c3[3 + 3 * hf + tt]);
+      const int act = (sm >> (3 * hf)) & 7;
+      if (act) {
+        st3m(x1n, offs + 24u * hf, act, xn0[0], xn0[1], xn0[2]);
+        st3m(x2n, offs + 24u * hf, act, xn1[0], xn1[1], xn1[2]);
+      }
+    }
+  }
+  // node-local terms of the stage output
+  const double sc = S == 4 ? A.h6 : 1.0;
+  double pre[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
+    pre[m] = c3[m] - sc * (minv * dg[m]);
+  }
+  double* const dst = (S == 4 ? A.xnew[2] : A.out) + so;
+  if (ua) st3(dst, offu, pre[0], pre[1], pre[2]);
+  if (sm == 0x3f) st6(dst, offs, &pre[3]);
+  else {
+    if (sm & 7) st3m(dst, offs, sm & 7, pre[3], pre[4], pre[5]);
+    if (sm & 56) st3m(dst, offs + 24u, (sm >> 3) & 7, pre[6], pre[7], pre[8]);
+  }
+  // flux records: (G_{x,ax}, G_{y,ax}, G_{z,ax}, w_{ax,x}, w_{ax,y}, w_{ax,z}), Gv[3 i + ax], wv[3 ax + i]
+  double* const Dp = A.D + (long long)shot * 18 * g.NN;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const unsigned o = d_rec(a, NN, n);
+    st2(Dp, o, Gv[a], Gv[3 + a]);
+    st2(Dp, o + 16u, Gv[6 + a], wv[3 * a]);
+    st2(Dp, o + 32u, wv[3 * a + 1], wv[3 * a + 2]);
+  }
+}
+
+// Pull of RK4 stage S at one PML node: the PML elements' terms from the flux
+// records along the three lattice lines through the node (side-scaled 5-point
+// stencils), the regular-domain K sums of an interface node from E, the
+// sources, then out = (node-local terms) + sc (src - M^-1 sums), sc = 1 or dt/6.
+template <int S, bool TR>
+__global__ void __launch_bounds__(SPLIT_NT, 4) pml_pull_kernel(const Stage3Args A) {
+  const Geo3& g = A.g;
+  const int lb = split_block(A.pnb);
+  if (lb >= A.pnb) return;
+  const int blk = A.pb0 + lb;
+  const int t = blk * SPLIT_NT + (int)threadIdx.x;
+  const unsigned ijk = A.pnodes[t];
+  if (ijk == 0xffffffffu) return;
+  const int shot = blockIdx.y;
+  const int i = ijk & 1023u, j = (ijk >> 10) & 1023u, k = ijk >> 20;
+  const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
+  const long long so = (long long)shot * A.shot_stride;
+  const unsigned NN = (unsigned)g.NN;
+  const unsigned SB = 8u * (3u * NN + (NN & 1u));
+  const unsigned offu = n * 24u, offs = SB + n * 48u;
+  const double* __restrict__ nodet = A.node;
+
+  const SpAxis ax3[3] = {sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
+                         sp_axis(k, g.nez, g.rlo[2], g.nez)};
+  const bool iface = sp_iface(ax3[0], ax3[1], ax3[2]);
+  const double* const Dp = A.D + (long long)shot * 18 * g.NN;
+  const int cc[3] = {i, j, k};
+  const int nax[3] = {g.NX, g.NY, g.NZ};
+  const int str[3] = {1, g.NX, g.NX * g.NY};
+  // per axis: su = sum_d cu[d] G_{.,ax}(n + (d - 2) str), sd the same with the
+  // s-row stencil (every offset but 0 negated) on w_{ax,.}
+  double su[3][3], sd[3][3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    double s0, s1, cu[5];
+    sp_side_scales(ax3[ax], ax3[ax == 0 ? 1 : 0], ax3[ax == 2 ? 1 : 2], s0, s1);
+    sp_stencil(ax3[ax].odd, s0, s1, cu);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { su[ax][c] = 0.0; sd[ax][c] = 0.0; }
+#pragma unroll
+    for (int d = 0; d < 5; ++d) {
+      // offsets outside the lattice have zero coefficients: read the node itself
+      const int p = cc[ax] + d - 2;
+      const unsigned m = (p >= 0 && p < nax[ax]) ? (unsigned)((int)n + (d - 2) * str[ax]) : n;
+      const unsigned o = d_rec(ax, NN, m);
+      double G0, G1, G2, W0, W1, W2;
+      ld2(Dp, o, G0, G1);
+      ld2(Dp, o + 16u, G2, W0);
+      ld2(Dp, o + 32u, W1, W2);
+      const double cw = d == 2 ? cu[d] : -cu[d];
+      su[ax][0] += cu[d] * G0; su[ax][1] += cu[d] * G1; su[ax][2] += cu[d] * G2;
+      sd[ax][0] += cw * W0; sd[ax][1] += cw * W1; sd[ax][2] += cw * W2;
+    }
+    // the sums are formed here, so that the loaded records die with the axis
+#pragma unroll
+    for (int c = 0; c < 3; ++c) asm volatile("" : "+v"(su[ax][c]), "+v"(sd[ax][c]));
+  }
+  double raw[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    raw[c] = g.detj * (((0.0 + g.dxi[0] * su[0][c]) + g.dxi[1] * su[1][c]) + g.dxi[2] * su[2][c]);
+  if (iface) {
+    double e0, e1, e2;
+    ld3(A.E + (long long)shot * 3 * g.NN, offu, e0, e1, e2);
+    raw[0] += e0; raw[1] += e1; raw[2] += e2;
+  }
+  double dd[3][3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dd[ax][c] = (g.detj * g.dxi[ax]) * sd[ax][c];
+  const double aE[6] = {dd[0][0], dd[1][1], dd[2][2], dd[1][0] + dd[0][1], dd[2][0] + dd[0][2], dd[2][1] + dd[1][2]};
+  if constexpr (!TR) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) raw[3 + m] = aE[m];
+  } else {
+    double lam, mu;
+    ld2(nodet, n * 16u, lam, mu);
+    const double l2m = lam + 2.0 * mu, m2 = 2.0 * mu;
+    raw[3] = -(l2m * aE[0] + lam * aE[1] + lam * aE[2]);
+    raw[4] = -(lam * aE[0] + l2m * aE[1] + lam * aE[2]);
+    raw[5] = -(lam * aE[0] + lam * aE[1] + l2m * aE[2]);
+    raw[6] = -(m2 * aE[3]);
+    raw[7] = -(m2 * aE[4]);
+    raw[8] = -(m2 * aE[5]);
+  }
+  // sources of this node (the entries of this block, a uniform range)
+  const bool ua = u_act3(g, i, j, k);
+  double src[3] = {0.0, 0.0, 0.0};
+  if (A.src_mode) {
+    const Src3& sp = A.src;
+    const int sdyn = A.step_dev ? *A.step_dev : 0;
+    const int pb = shot * sp.pnblk + blk;
+    const int e0 = sp.pblk_ptr[pb], e1 = sp.pblk_ptr[pb + 1];
+    for (int e = e0; e < e1; ++e) {
+      const int jj = sp.pblk_idx[e];
+      if (sp.node[jj] != (long long)n || !ua) continue;
+      const double mm = sp.m[jj];
+      double val;
+      if (A.src_mode == 1) {
+        const double f = sp.ftab[((size_t)shot * sp.nstep + (A.step + sdyn)) * 3 + A.fcol];
+        val = f * (sp.val[jj] * mm);
+      } else {
+        const size_t nr = (size_t)sp.nrec;
+        const size_t base = (size_t)shot * ((size_t)sp.nstep + 1) * nr + ((size_t)jj - (size_t)shot * nr);
+        const size_t ra = (size_t)(A.ra + sdyn);
+        const double va = sp.tr[base + ra * nr] - sp.obs[base + ra * nr];
+        val = A.ca * (va * mm);
+        if (A.rb_on) {
+          const size_t rb = (size_t)(A.rb + sdyn);
+          const double vb = sp.tr[base + rb * nr] - sp.obs[base + rb * nr];
+          val = val + A.cb * (vb * mm);
+        }
+      }
+      const int cpt = sp.comp[jj];
+      src[0] += cpt == 0 ? val : 0.0;
+      src[1] += cpt == 1 ? val : 0.0;
+      src[2] += cpt == 2 ? val : 0.0;
+    }
+  }
+  // finish on the node-local terms the nodal pass wrote
+  double* const dst = (S == 4 ? A.xnew[2] : A.out) + so;
+  const int sm = smask3(g, i, j, k);
+  double minv_u, minv_s;
+  ld2(nodet, 16u * NN + n * 32u, minv_u, minv_s);
+  const double sc = S == 4 ? A.h6 : 1.0;
+  double pre[9];
+  ld3(dst, offu, pre[0], pre[1], pre[2]);
+  ld2(dst, offs, pre[3], pre[4]);
+  ld2(dst, offs + 16u, pre[5], pre[6]);
+  ld2(dst, offs + 32u, pre[7], pre[8]);
+  double out[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
+    const double sv = m < 3 ? src[m] : 0.0;
+    out[m] = pre[m] + sc * (sv - minv * raw[m]);
+  }
+  if (ua) st3(dst, offu, out[0], out[1], out[2]);
+  if (sm == 0x3f) st6(dst, offs, &out[3]);
+  else {
+    if (sm & 7) st3m(dst, offs, sm & 7, out[3], out[4], out[5]);
+    if (sm & 56) st3m(dst, offs + 24u, (sm >> 3) & 7, out[6], out[7], out[8]);
+  }
+}
+
+template <int S, bool TR>
+hipError_t launch_split(const Stage3Args& A, int part, const int* tiles, int ntl, hipStream_t s) {
+  if (A.nshot <= 0) return hipSuccess;
+  if (part == 1) {
+    if (ntl <= 0) return hipSuccess;
+    const int nb = 8 * ((ntl * A.nshot + 7) / 8);
+    stage_kernel3<4, 3, true, false, S, TR, true><<<nb, NT3, 0, s>>>(A, tiles, ntl);
+  } else {
+    if (A.pnb <= 0) return hipSuccess;
+    const dim3 grid(8 * ((A.pnb + 7) / 8), A.nshot);
+    if (part == 0) pml_nodal_kernel<S, TR><<<grid, SPLIT_NT, 0, s>>>(A);
+    else pml_pull_kernel<S, TR><<<grid, SPLIT_NT, 0, s>>>(A);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_split_any(const Stage3Args& A, int stage, int transpose, int part, const int* tiles, int ntl,
+                            hipStream_t s) {
+  switch (stage * 2 + (transpose ? 1 : 0)) {
+    case 2: return launch_split<1, false>(A, part, tiles, ntl, s);
+    case 3: return launch_split<1, true>(A, part, tiles, ntl, s);
+    case 4: return launch_split<2, false>(A, part, tiles, ntl, s);
+    case 5: return launch_split<2, true>(A, part, tiles, ntl, s);
+    case 6: return launch_split<3, false>(A, part, tiles, ntl, s);
+    case 7: return launch_split<3, true>(A, part, tiles, ntl, s);
+    case 8: return launch_split<4, false>(A, part, tiles, ntl, s);
+    case 9: return launch_split<4, true>(A, part, tiles, ntl, s);
+    default: return hipErrorInvalidValue;
   }
 }
 
@@ -890,6 +1253,16 @@ hipError_t launch_stage3(const Stage3Args& A, int stage, int transpose, int cls,
     case 10: return launch3<5, false>(A, cls, tiles, ntl, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_split_nodal(const Stage3Args& A, int stage, int transpose, hipStream_t s) {
+  return launch_split_any(A, stage, transpose, 0, nullptr, 0, s);
+}
+hipError_t launch_split_rd(const Stage3Args& A, int stage, int transpose, const int* tiles, int ntl, hipStream_t s) {
+  return launch_split_any(A, stage, transpose, 1, tiles, ntl, s);
+}
+hipError_t launch_split_pull(const Stage3Args& A, int stage, int transpose, hipStream_t s) {
+  return launch_split_any(A, stage, transpose, 2, nullptr, 0, s);
 }
 
 }  // namespace hr

@@ -15,8 +15,9 @@ from tests.helpers import small_case, rel_l2
 pytestmark = pytest.mark.gpu
 
 OPS = {S.HR_K: 0, S.HR_C: 1, S.HR_G: 2}
-# stage-kernel configurations: (version, tile y-extent)
-KCFG = [(3, 3)]
+# stage-kernel configurations: (version, tile y-extent); 4 = split stage path
+# (nodal PML pass and pull beside the tiled regular-domain kernel), 3 = fused tile classes
+KCFG = [(3, 3), (4, 3)]
 
 
 def _abs_bound(A, x, y):
@@ -26,7 +27,7 @@ def _abs_bound(A, x, y):
 @pytest.mark.parametrize("shape,npml,top", [((8, 8, 6), 2, "reference"), ((7, 9, 5), (1, 2, 2, 1, 2), "none"),
                                             ((13, 10, 11), 3, "reference")])
 @pytest.mark.parametrize("transpose", [0, 1])
-@pytest.mark.parametrize("kcfg", KCFG)
+@pytest.mark.parametrize("kcfg", KCFG[:1])   # single-operator applies always use the fused kernel
 def test_apply_matches_oracle_csr(shape, npml, top, transpose, kcfg):
     npml = (npml,) * 5 if isinstance(npml, int) else npml
     mesh = BoxMesh(*shape, npml=npml, top_stress_bc=top)
@@ -151,7 +152,8 @@ def _observed(mesh, rec_eq, src_eq, src_val, nstep, dt, fr, seed):
     return U[:, rows].copy()
 
 
-@pytest.mark.parametrize("history,kcfg", [(S.HR_HISTORY_STORE, (3, 3)), (S.HR_HISTORY_CHECKPOINT, (3, 3))])
+@pytest.mark.parametrize("history,kcfg", [(S.HR_HISTORY_STORE, (3, 3)), (S.HR_HISTORY_CHECKPOINT, (3, 3)),
+                                          (S.HR_HISTORY_STORE, (4, 3)), (S.HR_HISTORY_CHECKPOINT, (4, 3))])
 def test_adjoint_gradient_matches_oracle(history, kcfg):
     mesh, model, rec_eq, src_eq, src_val = small_case(seed=1)
     nstep, dt, fr = 240, 1.0e-4, 60.0
