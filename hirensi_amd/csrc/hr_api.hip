@@ -25,7 +25,7 @@ using namespace hr;
 namespace {
 constexpr int DEFAULT_TZ = 7;   // element layers per z-chunk (sweep of 3..8 at 64x64x32: 7 first)
 constexpr int DEFAULT_TY = 3;
-constexpr int DEFAULT_VERSION = 3;   // 3 = fused tile classes, 4 = split stage path
+constexpr int DEFAULT_VERSION = 4;   // 4 = split stage path, 3 = fused tile classes
 constexpr int SPLIT_MAX_AXIS = 1024;  // lattice nodes per axis of the split path (packed node indices)
 
 struct HipError {
@@ -153,7 +153,7 @@ struct hr_ctx {
   // and interface sums of split_shots shots, source entries by pull block
   std::vector<std::pair<long long, int>> pnode_pos;   // (lattice index, list position), by lattice index
   unsigned* d_pnodes = nullptr;
-  int pnblk = 0, pnblk_bulk = 0;              // blocks of the list; the first pnblk_bulk hold non-interface nodes
+  int pnblk = 0;                              // SPLIT_NT-node blocks of the list
   int* d_tl_split = nullptr;
   int ntl_split = 0;
   double* d_D = nullptr;
@@ -486,21 +486,35 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
       B.pnodes = ctx->d_pnodes;
       B.D = ctx->d_D;
       B.E = ctx->d_E;
-      // all node blocks (the list holds the non-interface nodes, then the interface ones)
+      // all node blocks
       Stage3Args Ball = B;
       Ball.pb0 = 0;
       Ball.pnb = ctx->pnblk;
       if (ctx->concurrent) {
+        // the nodal pass on a side stream, the deferred per-step kernels on the
+        // other, the regular-domain kernel (the longer one) on the main stream,
+        // which then runs the pull once both side streams have joined
         HIPC(hipEventRecord(ctx->ev_fork, ctx->stream));
         HIPC(hipStreamWaitEvent(ctx->cstream[0], ctx->ev_fork, 0));
         if (prof) HIPC(hipEventRecord(ev.start[1], ctx->cstream[0]));
-        HIPC(launch_split_rd(B, A.stage, A.transpose, ctx->d_tl_split, ctx->ntl_split, ctx->cstream[0]));
+        HIPC(launch_split_nodal(Ball, A.stage, A.transpose, ctx->cstream[0]));
         if (prof) HIPC(hipEventRecord(ev.end[1], ctx->cstream[0]));
         HIPC(hipEventRecord(ctx->ev_join[0], ctx->cstream[0]));
+        const bool side_work = !ctx->deferred.empty();
+        if (side_work) {
+          // the deferred kernels read the state the stage reads and write their
+          // own outputs only; they queue on the context's stream, here the second side stream
+          HIPC(hipStreamWaitEvent(ctx->cstream[1], ctx->ev_fork, 0));
+          std::swap(ctx->stream, ctx->cstream[1]);
+          const int rc = run_deferred(ctx);
+          std::swap(ctx->stream, ctx->cstream[1]);
+          if (rc) return rc;
+          HIPC(hipEventRecord(ctx->ev_join[1], ctx->cstream[1]));
+        }
         if (prof) HIPC(hipEventRecord(ev.start[0], ctx->stream));
-        HIPC(launch_split_nodal(Ball, A.stage, A.transpose, ctx->stream));
-        if (int rc = run_deferred(ctx)) return rc;
+        HIPC(launch_split_rd(B, A.stage, A.transpose, ctx->d_tl_split, ctx->ntl_split, ctx->stream));
         HIPC(hipStreamWaitEvent(ctx->stream, ctx->ev_join[0], 0));
+        if (side_work) HIPC(hipStreamWaitEvent(ctx->stream, ctx->ev_join[1], 0));
         HIPC(launch_split_pull(Ball, A.stage, A.transpose, ctx->stream));
         if (prof) HIPC(hipEventRecord(ev.end[0], ctx->stream));
       } else {
@@ -564,7 +578,7 @@ int setup_tiles(hr_ctx* ctx) {
   zb.push_back(g.nez);
   if ((int)zb.size() - 1 > HR_MAX_ZCHUNKS) return fail(ctx, 1, "too many z-chunks (%d); raise tz", (int)zb.size() - 1);
   g.tiles[2] = (int)zb.size() - 1;
-  std::vector<int> lists[3];
+  std::vector<int> lists[3], rd_any;
   const int nt = ntiles(g);
   // list order: x fastest, then z, then y, so that the z-neighbour whose top slab is a
   // chunk's halo slab runs on the same XCD soon after (measured 2 % faster than x, y, z)
@@ -579,6 +593,7 @@ int setup_tiles(hr_ctx* ctx) {
           if (rd) has_rd = true; else has_pml = true;
         }
     lists[(has_rd && has_pml) ? 0 : (has_rd ? 1 : 2)].push_back(b);
+    if (has_rd) rd_any.push_back(b);
   }
   // Each XCD (blockIdx mod 8) takes a contiguous run of a class list.  A Morton order of the
   // runs measured more fabric traffic.
@@ -586,13 +601,11 @@ int setup_tiles(hr_ctx* ctx) {
     ctx->ntl[c] = (int)lists[c].size();
     if (int rc = upload(ctx, &ctx->d_tl[c], lists[c])) return rc;
   }
-  // the split path's regular-domain kernel runs every tile with a regular-domain element
-  {
-    std::vector<int> sl(lists[0]);
-    sl.insert(sl.end(), lists[1].begin(), lists[1].end());
-    ctx->ntl_split = (int)sl.size();
-    if (int rc = upload(ctx, &ctx->d_tl_split, sl)) return rc;
-  }
+  // the split path's regular-domain kernel runs every tile with a regular-domain
+  // element, in the same traversal order, so that each XCD's contiguous run mixes
+  // interface and interior tiles as the domain does
+  ctx->ntl_split = (int)rd_any.size();
+  if (int rc = upload(ctx, &ctx->d_tl_split, rd_any)) return rc;
   // per-tile source lists of an earlier forward no longer match the tiles
   ctx->have_src = false;
   ctx->f_nstep = 0;
@@ -890,26 +903,18 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
   // PML nodes of the split stage path, in lattice order; lattices wider than its
   // packed indices use the fused tile classes
   if (g.NX <= SPLIT_MAX_AXIS && g.NY <= SPLIT_MAX_AXIS && g.NZ <= SPLIT_MAX_AXIS && 144.0 * (double)g.NN < 4294967296.0) {
-    // non-interface PML nodes, then those a regular-domain element also holds
-    // (their pull waits for the regular-domain kernel), each part in lattice order
-    // and padded to whole blocks
+    // lattice order (measured 20 % faster per gradient than the interface nodes
+    // listed last), padded to whole blocks
     std::vector<unsigned> pk;
-    for (int part = 0; part < 2; ++part) {
-      for (int k = 0; k < g.NZ; ++k)
-        for (int j = 0; j < g.NY; ++j)
-          for (int i = 0; i < g.NX; ++i) {
-            if (!h_pml_node(g, i, j, k)) continue;
-            const bool iface = sp_iface(sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
-                                        sp_axis(k, g.nez, g.rlo[2], g.nez));
-            if (iface != (part == 1)) continue;
-            ctx->pnode_pos.push_back({lat(g, i, j, k), (int)pk.size()});
-            pk.push_back((unsigned)i | ((unsigned)j << 10) | ((unsigned)k << 20));
-          }
-      while (pk.size() % SPLIT_NT) pk.push_back(0xffffffffu);
-      if (part == 0) ctx->pnblk_bulk = (int)(pk.size() / SPLIT_NT);
-    }
+    for (int k = 0; k < g.NZ; ++k)
+      for (int j = 0; j < g.NY; ++j)
+        for (int i = 0; i < g.NX; ++i) {
+          if (!h_pml_node(g, i, j, k)) continue;
+          ctx->pnode_pos.push_back({lat(g, i, j, k), (int)pk.size()});
+          pk.push_back((unsigned)i | ((unsigned)j << 10) | ((unsigned)k << 20));
+        }
+    while (pk.size() % SPLIT_NT) pk.push_back(0xffffffffu);
     ctx->pnblk = (int)(pk.size() / SPLIT_NT);
-    std::sort(ctx->pnode_pos.begin(), ctx->pnode_pos.end());
     if (int rc = upload(ctx, &ctx->d_pnodes, pk)) return rc;
   } else {
     ctx->version = 3;

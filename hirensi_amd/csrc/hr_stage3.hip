@@ -1051,7 +1051,7 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
 // stencils), the regular-domain K sums of an interface node from E, the
 // sources, then out = (node-local terms) + sc (src - M^-1 sums), sc = 1 or dt/6.
 template <int S, bool TR>
-__global__ void __launch_bounds__(SPLIT_NT, 4) pml_pull_kernel(const Stage3Args A) {
+__global__ void __launch_bounds__(SPLIT_NT, 3) pml_pull_kernel(const Stage3Args A) {
   const Geo3& g = A.g;
   const int lb = split_block(A.pnb);
   if (lb >= A.pnb) return;
@@ -1068,6 +1068,17 @@ __global__ void __launch_bounds__(SPLIT_NT, 4) pml_pull_kernel(const Stage3Args 
   const unsigned offu = n * 24u, offs = SB + n * 48u;
   const double* __restrict__ nodet = A.node;
 
+  // node-local terms the nodal pass wrote, and M^-1, issued with the records
+  double* const dst = (S == 4 ? A.xnew[2] : A.out) + so;
+  double minv_u, minv_s;
+  ld2(nodet, 16u * NN + n * 32u, minv_u, minv_s);
+  double pre[9];
+  ld3(dst, offu, pre[0], pre[1], pre[2]);
+  ld2(dst, offs, pre[3], pre[4]);
+  ld2(dst, offs + 16u, pre[5], pre[6]);
+  ld2(dst, offs + 32u, pre[7], pre[8]);
+  double lam = 0.0, mu = 0.0;
+  if constexpr (TR) ld2(nodet, n * 16u, lam, mu);
   const SpAxis ax3[3] = {sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
                          sp_axis(k, g.nez, g.rlo[2], g.nez)};
   const bool iface = sp_iface(ax3[0], ax3[1], ax3[2]);
@@ -1087,6 +1098,9 @@ __global__ void __launch_bounds__(SPLIT_NT, 4) pml_pull_kernel(const Stage3Args 
     for (int c = 0; c < 3; ++c) { su[ax][c] = 0.0; sd[ax][c] = 0.0; }
 #pragma unroll
     for (int d = 0; d < 5; ++d) {
+      // odd coordinates have no +-2 terms: skipped in y and z, where a row of
+      // consecutive list entries shares the coordinate
+      if (ax > 0 && (d == 0 || d == 4) && ax3[ax].odd) continue;
       // offsets outside the lattice have zero coefficients: read the node itself
       const int p = cc[ax] + d - 2;
       const unsigned m = (p >= 0 && p < nax[ax]) ? (unsigned)((int)n + (d - 2) * str[ax]) : n;
@@ -1099,9 +1113,13 @@ __global__ void __launch_bounds__(SPLIT_NT, 4) pml_pull_kernel(const Stage3Args 
       su[ax][0] += cu[d] * G0; su[ax][1] += cu[d] * G1; su[ax][2] += cu[d] * G2;
       sd[ax][0] += cw * W0; sd[ax][1] += cw * W1; sd[ax][2] += cw * W2;
     }
-    // the sums are formed here, so that the loaded records die with the axis
+    // the x and y sums are formed before the z records are loaded (register budget)
+    if (ax == 1) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) asm volatile("" : "+v"(su[ax][c]), "+v"(sd[ax][c]));
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) asm volatile("" : "+v"(su[a][c]), "+v"(sd[a][c]));
+    }
   }
   double raw[9];
 #pragma unroll
@@ -1122,8 +1140,6 @@ __global__ void __launch_bounds__(SPLIT_NT, 4) pml_pull_kernel(const Stage3Args 
 #pragma unroll
     for (int m = 0; m < 6; ++m) raw[3 + m] = aE[m];
   } else {
-    double lam, mu;
-    ld2(nodet, n * 16u, lam, mu);
     const double l2m = lam + 2.0 * mu, m2 = 2.0 * mu;
     raw[3] = -(l2m * aE[0] + lam * aE[1] + lam * aE[2]);
     raw[4] = -(lam * aE[0] + l2m * aE[1] + lam * aE[2]);
@@ -1166,17 +1182,9 @@ __global__ void __launch_bounds__(SPLIT_NT, 4) pml_pull_kernel(const Stage3Args 
       src[2] += cpt == 2 ? val : 0.0;
     }
   }
-  // finish on the node-local terms the nodal pass wrote
-  double* const dst = (S == 4 ? A.xnew[2] : A.out) + so;
+  // finish on the node-local terms
   const int sm = smask3(g, i, j, k);
-  double minv_u, minv_s;
-  ld2(nodet, 16u * NN + n * 32u, minv_u, minv_s);
   const double sc = S == 4 ? A.h6 : 1.0;
-  double pre[9];
-  ld3(dst, offu, pre[0], pre[1], pre[2]);
-  ld2(dst, offs, pre[3], pre[4]);
-  ld2(dst, offs + 16u, pre[5], pre[6]);
-  ld2(dst, offs + 32u, pre[7], pre[8]);
   double out[9];
 #pragma unroll
   for (int m = 0; m < 9; ++m) {

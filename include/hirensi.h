@@ -257,10 +257,10 @@ int hr_get_history_plan(hr_ctx* ctx, int nstep, int nshot, int history, int segm
 int hr_revolve_schedule(int nseg, int nsnap, int max_actions, int* actions, int* nactions, long long* advances);
 
 /* Stage-kernel configuration: tile of 4 x ty elements in x-y (ty = 3), tz
- * elements per z-chunk, kernel version (0 keeps it): 3 (the default) runs
- * the fused tile classes (mixed, regular-domain, PML) for every stage; 4
- * splits each RK4 stage into a nodal pass over the PML nodes, the tiled
- * regular-domain kernel and a nodal pull over the PML nodes.  Single-operator
+ * elements per z-chunk, kernel version (0 keeps it): 4 (the default) splits
+ * each RK4 stage into a nodal pass over the PML nodes, the tiled
+ * regular-domain kernel and a nodal pull over the PML nodes; 3 runs the fused
+ * tile classes (mixed, regular-domain, PML) for every stage.  Single-operator
  * applies and the central-difference step always use the fused kernel.  The
  * kernels of a stage run concurrently on side streams unless
  * HR_TILING_SERIAL_CLASSES is or-ed into version.  Invalidates the history of
