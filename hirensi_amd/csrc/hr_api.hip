@@ -26,7 +26,6 @@ namespace {
 constexpr int DEFAULT_TZ = 7;   // element layers per z-chunk (sweep of 3..8 at 64x64x32: 7 first)
 constexpr int DEFAULT_TY = 3;
 constexpr int DEFAULT_VERSION = 4;   // 4 = split stage path, 3 = fused tile classes
-constexpr int SPLIT_MAX_AXIS = 1024;  // lattice nodes per axis of the split path (packed node indices)
 
 struct HipError {
   hipError_t e;
@@ -151,9 +150,9 @@ struct hr_ctx {
   // k << 20, and their lattice indices on the host), the tile list of its
   // regular-domain kernel (the mixed and regular-domain tiles), the flux records
   // and interface sums of split_shots shots, source entries by pull block
-  std::vector<std::pair<long long, int>> pnode_pos;   // (lattice index, list position), by lattice index
-  unsigned* d_pnodes = nullptr;
-  int pnblk = 0;                              // SPLIT_NT-node blocks of the list
+  std::vector<std::pair<long long, int>> pnode_pos;   // (lattice index, position in lattice order), by lattice index
+  bool split_ok = false;                      // the lattice fits the split path
+  int pnblk = 0;                              // SPLIT_NT-node blocks of the PML nodes
   int* d_tl_split = nullptr;
   int ntl_split = 0;
   double* d_D = nullptr;
@@ -481,9 +480,8 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
       // nodal pass (main stream) beside the regular-domain kernel (side stream),
       // then the nodal pull, which reads the interface sums of the latter
       Stage3Args B = args3(ctx, A, sp);
-      if (ctx->split_shots < B.nshot || !ctx->d_D || !ctx->d_pnodes)
+      if (ctx->split_shots < B.nshot || !ctx->d_D || !ctx->split_ok)
         return fail(ctx, 3, "split stage path without its buffers for %d shots", B.nshot);
-      B.pnodes = ctx->d_pnodes;
       B.D = ctx->d_D;
       B.E = ctx->d_E;
       // all node blocks
@@ -900,22 +898,21 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
     HIPC(hipEventCreateWithFlags(&ctx->ev_join[q], hipEventDisableTiming));
   }
   HIPC(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
-  // PML nodes of the split stage path, in lattice order; lattices wider than its
-  // packed indices use the fused tile classes
-  if (g.NX <= SPLIT_MAX_AXIS && g.NY <= SPLIT_MAX_AXIS && g.NZ <= SPLIT_MAX_AXIS && 144.0 * (double)g.NN < 4294967296.0) {
-    // lattice order (measured 20 % faster per gradient than the interface nodes
-    // listed last), padded to whole blocks
-    std::vector<unsigned> pk;
+  // PML nodes of the split stage path; lattices whose flux records pass 32-bit
+  // byte offsets use the fused tile classes
+  if (144.0 * (double)g.NN < 4294967296.0) {
+    // the nodal kernels walk the PML nodes in lattice order (sp_pml_node; measured
+    // 20 % faster per gradient than with the interface nodes last); their
+    // positions in that order, for the pull's per-block source lists
+    int pos = 0;
     for (int k = 0; k < g.NZ; ++k)
       for (int j = 0; j < g.NY; ++j)
-        for (int i = 0; i < g.NX; ++i) {
-          if (!h_pml_node(g, i, j, k)) continue;
-          ctx->pnode_pos.push_back({lat(g, i, j, k), (int)pk.size()});
-          pk.push_back((unsigned)i | ((unsigned)j << 10) | ((unsigned)k << 20));
-        }
-    while (pk.size() % SPLIT_NT) pk.push_back(0xffffffffu);
-    ctx->pnblk = (int)(pk.size() / SPLIT_NT);
-    if (int rc = upload(ctx, &ctx->d_pnodes, pk)) return rc;
+        for (int i = 0; i < g.NX; ++i)
+          if (h_pml_node(g, i, j, k)) ctx->pnode_pos.push_back({lat(g, i, j, k), pos++});
+    int gl[3] = {g.rlo[0], g.rlo[1], g.rlo[2]}, gh[2] = {g.rhi[0], g.rhi[1]};
+    if ((unsigned)pos != sp_pml_count(g.NX, g.NY, g.NZ, gl, gh)) return fail(ctx, 3, "PML node count mismatch");
+    ctx->pnblk = (pos + SPLIT_NT - 1) / SPLIT_NT;
+    ctx->split_ok = true;
   } else {
     ctx->version = 3;
   }
@@ -968,7 +965,7 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   F(ctx->d_obs_sorted);
   F(ctx->d_img); F(ctx->d_restr); F(ctx->d_reg); F(ctx->d_cost_el); F(ctx->d_red); F(ctx->d_pm); F(ctx->d_pg);
   for (int c = 0; c < 3; ++c) F(ctx->d_tl[c]);
-  F(ctx->d_pnodes); F(ctx->d_tl_split); F(ctx->d_D); F(ctx->d_E);
+  F(ctx->d_tl_split); F(ctx->d_D); F(ctx->d_E);
   F(ctx->d_psrc_ptr); F(ctx->d_psrc_idx); F(ctx->d_prec_ptr); F(ctx->d_prec_idx);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
@@ -1205,7 +1202,7 @@ int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   version &= ~HR_TILING_SERIAL_CLASSES;
   if (ty < 0 || tz < 0 || (version != 0 && version != 3 && version != 4))
     return fail(ctx, 1, "bad tiling (stage kernel version 3 or 4)");
-  if (version == 4 && !ctx->d_pnodes) return fail(ctx, 1, "lattice too large for the split stage path (version 4)");
+  if (version == 4 && !ctx->split_ok) return fail(ctx, 1, "lattice too large for the split stage path (version 4)");
   Geo& g = ctx->g;
   ctx->concurrent = serial ? 0 : 1;
   const int t = ty ? ty : DEFAULT_TY;
@@ -1451,7 +1448,7 @@ int forward_impl(hr_ctx* ctx, int nshot, int nstep, double dt, const double* fta
     std::memset(&h3, 0, sizeof(h3));
     h3.tile_ptr = ctx->d_src_tile_ptr; h3.node = ctx->d_src_node; h3.comp = ctx->d_src_comp;
     h3.m = ctx->d_src_m; h3.val = ctx->d_src_val; h3.ftab = ctx->d_ftab; h3.nstep = nstep; h3.ntiles = nt;
-    if (ctx->d_pnodes)
+    if (ctx->split_ok)
       if (int rc = build_pull_blocks(ctx, ptr, node, nshot, nt, &ctx->d_psrc_ptr, &ctx->d_psrc_idx, h3)) return rc;
     ctx->src3_fwd = h3;
     ctx->have_src = true;
@@ -1726,7 +1723,7 @@ int hr_adjoint_gradient(hr_ctx* ctx, int nstep, double dt, int nrec, const long 
     h3.tile_ptr = ctx->d_rec_tile_ptr; h3.node = ctx->d_rec_node; h3.comp = ctx->d_rec_comp;
     h3.m = ctx->d_rec_m; h3.tr = ctx->d_tr_sorted; h3.obs = ctx->d_obs_sorted; h3.nrec = nrec;
     h3.nstep = nstep; h3.ntiles = ntiles(ctx->g);
-    if (ctx->d_pnodes)
+    if (ctx->split_ok)
       if (int rc = build_pull_blocks(ctx, rptr, rnode, nshot, h3.ntiles, &ctx->d_prec_ptr, &ctx->d_prec_idx, h3))
         return rc;
     ctx->src3_adj = h3;

@@ -149,11 +149,10 @@ struct Stage3Args {
   int nshot;                       // shots in the launch; state vectors are shot-major
   long long shot_stride;           // elements per shot of a state vector (9 NN)
   double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed); stage 5: dt^2, dt, dt^3
-  // split stage path (RK4 stages): the PML node list (packed i | j << 10 | k <<
-  // 20; 0xffffffff pads), of which a nodal launch covers the SPLIT_NT-node
-  // blocks [pb0, pb0 + pnb); the flux records D (18 NN doubles per shot) and the
-  // regular-domain K sums of interface nodes E (3 NN doubles per shot)
-  const unsigned* pnodes;
+  // split stage path (RK4 stages): a nodal launch covers the SPLIT_NT-node
+  // blocks [pb0, pb0 + pnb) of the PML nodes in lattice order; the flux records
+  // D (18 NN doubles per shot) and the regular-domain K sums of interface nodes
+  // E (3 NN doubles per shot)
   int pb0, pnb;
   double* D;
   double* E;

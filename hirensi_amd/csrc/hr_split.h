@@ -74,4 +74,52 @@ constexpr bool sp_iface(const SpAxis& X, const SpAxis& Y, const SpAxis& Z) {
   return (X.r[0] + X.r[1]) * (Y.r[0] + Y.r[1]) * (Z.r[0] + Z.r[1]) > 0;
 }
 
+// The PML nodes in lattice order (i fastest, then j, then k) without a list:
+// the planes k <= 2 lo[2] are whole PML planes; every higher plane lacks the
+// regular-domain interior i in (2 lo[0], 2 hi[0]), j in (2 lo[1], 2 hi[1]), so its
+// rows with j in that range hold the two runs i <= 2 lo[0] and i >= 2 hi[0].
+// Entry t of that order -> (i, j, k); returns false past the last PML node.
+constexpr bool sp_pml_node(unsigned t, int NX, int NY, int NZ, const int lo[3], const int hi[2], int& i, int& j,
+                           int& k) {
+  const unsigned nxy = (unsigned)NX * (unsigned)NY;
+  const unsigned ax = (unsigned)(2 * (hi[0] - lo[0]) - 1), ay = (unsigned)(2 * (hi[1] - lo[1]) - 1);
+  const unsigned P = nxy - ax * ay;                       // PML nodes of a holed plane
+  const unsigned F = (unsigned)(2 * lo[2] + 1) * nxy;     // nodes of the whole planes
+  if (t < F) {
+    k = (int)(t / nxy);
+    const unsigned r = t - (unsigned)k * nxy;
+    j = (int)(r / (unsigned)NX);
+    i = (int)(r - (unsigned)j * (unsigned)NX);
+    return true;
+  }
+  const unsigned u = t - F;
+  const unsigned kk = u / P;
+  if ((int)kk >= NZ - 2 * lo[2] - 1) return false;
+  k = 2 * lo[2] + 1 + (int)kk;
+  const unsigned r = u - kk * P;
+  const unsigned R0 = (unsigned)(2 * lo[1] + 1) * (unsigned)NX;   // whole rows below the interior
+  const unsigned W = (unsigned)NX - ax;                            // nodes of a holed row
+  const unsigned H = ay * W;
+  if (r < R0) {
+    j = (int)(r / (unsigned)NX);
+    i = (int)(r - (unsigned)j * (unsigned)NX);
+  } else if (r - R0 < H) {
+    const unsigned r2 = r - R0, jr = r2 / W, q = r2 - jr * W;
+    j = 2 * lo[1] + 1 + (int)jr;
+    i = (int)(q <= (unsigned)(2 * lo[0]) ? q : q + ax);
+  } else {
+    const unsigned r3 = r - R0 - H, jr = r3 / (unsigned)NX;
+    j = 2 * hi[1] + (int)jr;
+    i = (int)(r3 - jr * (unsigned)NX);
+  }
+  return true;
+}
+
+// number of PML nodes
+constexpr unsigned sp_pml_count(int NX, int NY, int NZ, const int lo[3], const int hi[2]) {
+  const unsigned nxy = (unsigned)NX * (unsigned)NY;
+  const unsigned ax = (unsigned)(2 * (hi[0] - lo[0]) - 1), ay = (unsigned)(2 * (hi[1] - lo[1]) - 1);
+  return (unsigned)(2 * lo[2] + 1) * nxy + (unsigned)(NZ - 2 * lo[2] - 1) * (nxy - ax * ay);
+}
+
 }  // namespace hr

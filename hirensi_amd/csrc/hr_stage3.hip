@@ -877,8 +877,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
 }
 
 // ------------------------------------------------------------ split stage path
-// Blocks of SPLIT_NT consecutive entries of the PML node list (entries past the
-// nodes are 0xffffffff), one node per thread.  XCD x = blockIdx % 8 takes the
+// Blocks of SPLIT_NT consecutive PML nodes in lattice order (sp_pml_node), one
+// node per thread.  XCD x = blockIdx % 8 takes the
 // logical blocks [x C, (x + 1) C) of the launch, so that each XCD's L2 holds a
 // contiguous run of the list (the neighbouring rows and planes of its nodes); a
 // launch has 8 C blocks and starts at logical block A.pb0.
@@ -909,11 +909,9 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
   const Geo3& g = A.g;
   const int lb = split_block(A.pnb);
   if (lb >= A.pnb) return;
-  const int t = (A.pb0 + lb) * SPLIT_NT + (int)threadIdx.x;
-  const unsigned ijk = A.pnodes[t];
-  if (ijk == 0xffffffffu) return;
+  int i, j, k;
+  if (!sp_pml_node((unsigned)((A.pb0 + lb) * SPLIT_NT) + threadIdx.x, g.NX, g.NY, g.NZ, g.rlo, g.rhi, i, j, k)) return;
   const int shot = blockIdx.y;
-  const int i = ijk & 1023u, j = (ijk >> 10) & 1023u, k = ijk >> 20;
   const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
   const long long so = (long long)shot * A.shot_stride;
   const unsigned NN = (unsigned)g.NN;
@@ -1056,11 +1054,9 @@ __global__ void __launch_bounds__(SPLIT_NT, 3) pml_pull_kernel(const Stage3Args 
   const int lb = split_block(A.pnb);
   if (lb >= A.pnb) return;
   const int blk = A.pb0 + lb;
-  const int t = blk * SPLIT_NT + (int)threadIdx.x;
-  const unsigned ijk = A.pnodes[t];
-  if (ijk == 0xffffffffu) return;
+  int i, j, k;
+  if (!sp_pml_node((unsigned)(blk * SPLIT_NT) + threadIdx.x, g.NX, g.NY, g.NZ, g.rlo, g.rhi, i, j, k)) return;
   const int shot = blockIdx.y;
-  const int i = ijk & 1023u, j = (ijk >> 10) & 1023u, k = ijk >> 20;
   const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
   const long long so = (long long)shot * A.shot_stride;
   const unsigned NN = (unsigned)g.NN;
@@ -1079,6 +1075,10 @@ __global__ void __launch_bounds__(SPLIT_NT, 3) pml_pull_kernel(const Stage3Args 
   ld2(dst, offs + 32u, pre[7], pre[8]);
   double lam = 0.0, mu = 0.0;
   if constexpr (TR) ld2(nodet, n * 16u, lam, mu);
+  // interface sums, read for every node with the first batch and used (by a
+  // select) only where a regular-domain element holds the node
+  double e3[3];
+  ld3(A.E + (long long)shot * 3 * g.NN, offu, e3[0], e3[1], e3[2]);
   const SpAxis ax3[3] = {sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
                          sp_axis(k, g.nez, g.rlo[2], g.nez)};
   const bool iface = sp_iface(ax3[0], ax3[1], ax3[2]);
@@ -1125,11 +1125,8 @@ __global__ void __launch_bounds__(SPLIT_NT, 3) pml_pull_kernel(const Stage3Args 
 #pragma unroll
   for (int c = 0; c < 3; ++c)
     raw[c] = g.detj * (((0.0 + g.dxi[0] * su[0][c]) + g.dxi[1] * su[1][c]) + g.dxi[2] * su[2][c]);
-  if (iface) {
-    double e0, e1, e2;
-    ld3(A.E + (long long)shot * 3 * g.NN, offu, e0, e1, e2);
-    raw[0] += e0; raw[1] += e1; raw[2] += e2;
-  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) raw[c] += iface ? e3[c] : 0.0;
   double dd[3][3];
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax)

@@ -80,5 +80,30 @@ int main() {
         }
   }
   std::printf("nodes %ld interface %ld max diff %.3e\n", checked, ifaces, worst);
+  // list-free enumeration of the PML nodes in lattice order
+  long listed = 0;
+  for (const Box& B : boxes) {
+    const int N[3] = {2 * B.ne[0] + 1, 2 * B.ne[1] + 1, 2 * B.ne[2] + 1};
+    unsigned t = 0;
+    for (int k = 0; k < N[2]; ++k)
+      for (int j = 0; j < N[1]; ++j)
+        for (int i = 0; i < N[0]; ++i) {
+          const bool rdi = i > 2 * B.rlo[0] && i < 2 * B.rhi[0] && j > 2 * B.rlo[1] && j < 2 * B.rhi[1] && k > 2 * B.rlo[2];
+          if (rdi) continue;
+          int a = -1, b = -1, q = -1;
+          if (!sp_pml_node(t, N[0], N[1], N[2], B.rlo, B.rhi, a, b, q) || a != i || b != j || q != k) {
+            std::printf("PML node %u: got (%d %d %d), expected (%d %d %d)\n", t, a, b, q, i, j, k);
+            return 1;
+          }
+          ++t;
+          ++listed;
+        }
+    int a, b, q;
+    if (sp_pml_node(t, N[0], N[1], N[2], B.rlo, B.rhi, a, b, q) || t != sp_pml_count(N[0], N[1], N[2], B.rlo, B.rhi)) {
+      std::printf("PML node count differs\n");
+      return 1;
+    }
+  }
+  std::printf("PML nodes enumerated %ld\n", listed);
   return (worst <= 1e-14 && checked > 0 && ifaces > 0) ? 0 : 1;
 }
