@@ -1049,7 +1049,7 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
 // stencils), the regular-domain K sums of an interface node from E, the
 // sources, then out = (node-local terms) + sc (src - M^-1 sums), sc = 1 or dt/6.
 template <int S, bool TR>
-__global__ void __launch_bounds__(SPLIT_NT, 3) pml_pull_kernel(const Stage3Args A) {
+__global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args A) {
   const Geo3& g = A.g;
   const int lb = split_block(A.pnb);
   if (lb >= A.pnb) return;
@@ -1112,13 +1112,6 @@ __global__ void __launch_bounds__(SPLIT_NT, 3) pml_pull_kernel(const Stage3Args 
       const double cw = d == 2 ? cu[d] : -cu[d];
       su[ax][0] += cu[d] * G0; su[ax][1] += cu[d] * G1; su[ax][2] += cu[d] * G2;
       sd[ax][0] += cw * W0; sd[ax][1] += cw * W1; sd[ax][2] += cw * W2;
-    }
-    // the x and y sums are formed before the z records are loaded (register budget)
-    if (ax == 1) {
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) asm volatile("" : "+v"(su[a][c]), "+v"(sd[a][c]));
     }
   }
   double raw[9];
