@@ -94,10 +94,41 @@ def stage_bytes(neq, nnode):
     return b
 
 
-def measured_traffic():
+def split_stage_bytes(mesh, transpose=False):
+    """Algorithmic bytes of one RK4 stage s of the split stage path (the default):
+    the nodal pass over the n_p PML nodes reads the 2+s raw inputs of all nine
+    components and six node-table doubles, writes the 18 flux-record doubles and
+    the nine node-local outputs (stage 4: also x1, x2); the nodal pull reads the
+    flux records (each once), the node-local outputs, M^-1, the interface sums
+    (and lambda, mu when transposed) and writes nine outputs; the regular-domain
+    kernel reads, per regular-domain-only node, the displacement of the raw inputs
+    that enter its stage input (1, 2, 3, 3 vectors; stage 4: all six for the
+    update), lambda, mu and M^-1, and writes three outputs (stage 4: nine)."""
+    lo, hi = mesh.rd_lo, mesh.rd_hi
+    nxy = mesh.NX * mesh.NY
+    ax, ay = 2 * (hi[0] - lo[0]) - 1, 2 * (hi[1] - lo[1]) - 1
+    n_p = (2 * lo[2] + 1) * nxy + (mesh.NZ - 2 * lo[2] - 1) * (nxy - ax * ay)
+    n_r = mesh.nnode - n_p
+    b = {}
+    for st in (1, 2, 3, 4):
+        k1 = n_p * (9 * (2 + st) + 6 + 18 + 9 + (18 if st == 4 else 0))
+        k3 = n_p * (18 + 9 + 2 + 3 + (2 if transpose else 0) + 9)
+        nv = 6 if st == 4 else {1: 1, 2: 2, 3: 3}[st]
+        k2 = n_r * (3 * nv + 2 + 2 + (9 if st == 4 else 3))
+        b[st] = 8.0 * (k1 + k2 + k3)
+    return b
+
+
+def traffic_file(split):
+    return os.path.join("profiles", "split_stage_traffic_64x64x32.json" if split else
+                        "stage_kernel_traffic_64x64x32.json")
+
+
+def measured_traffic(split):
     """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "stage_kernel_traffic_64x64x32.json")
+    WRITE_SIZE summary of this workload and stage path (copy-kernel
+    calibrated), or None."""
+    path = os.path.join(ROOT, traffic_file(split))
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -581,14 +612,20 @@ def main():
     value = dof_steps / wall
     # roofline of the stage kernel: algorithmic bytes per launch (a quarter of B_step) /
     # average launch duration; the fused kernel's own byte count is reported beside it
-    sb = stage_bytes(neq, mesh.nnode)
+    split = ctx.version() == 4
+    if split:
+        # forward and adjoint stages in equal numbers
+        sf, sa = split_stage_bytes(mesh), split_stage_bytes(mesh, transpose=True)
+        sb = {st: 0.5 * (sf[st] + sa[st]) for st in (1, 2, 3, 4)}
+    else:
+        sb = stage_bytes(neq, mesh.nnode)
     launches = int(n_l[1:].sum())
     kern_ms = float(ms_l[1:].sum())
     alg_per_launch = step_bytes(neq, mesh.nnode) / 4.0
     fused_bytes = sum(sb[s] * n_l[s] for s in (1, 2, 3, 4))
     avg_us = kern_ms * 1e3 / launches
     achieved = alg_per_launch * launches / (kern_ms * 1e-3) / 1e9
-    tr = measured_traffic()
+    tr = measured_traffic(split)
     out = {
         "metric": "fwd+adjoint gradient DOF-steps/s (64x64x32, 4000 steps, 1 GPU)",
         "value": value,
@@ -609,13 +646,16 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": tr["traffic_bytes_per_launch"] if tr else None,
-                     "traffic_source": "profiles/stage_kernel_traffic_64x64x32.json" if tr else None,
-                     "kernel": "stage kernel (all tile classes of one RK4 stage)", "launches": launches,
+                     "traffic_source": traffic_file(split) if tr else None,
+                     "kernel": ("RK4 stage, split path: nodal PML pass beside the regular-domain tile kernel, "
+                                "then the nodal PML pull" if split else
+                                "stage kernel (all tile classes of one RK4 stage)"),
+                     "launches": launches,
                      "avg_launch_us": avg_us,
                      "alg_bytes_per_launch": alg_per_launch,
                      "alg_bytes_convention": "B_step/4, B_step = 8 (36 N + 8 N_node)",
-                     "fused_bytes_per_launch": fused_bytes / launches,
-                     "fused_achieved": fused_bytes / (kern_ms * 1e-3) / 1e9,
+                     "own_bytes_per_launch": fused_bytes / launches,
+                     "own_achieved": fused_bytes / (kern_ms * 1e-3) / 1e9,
                      "stage_kernel_share_of_wall": kern_ms * 1e-3 / (wall * a.steps)},
     }
     if a.graph_leg > 0:

@@ -1215,6 +1215,14 @@ int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   return setup_tiles(ctx);
 }
 
+int hr_get_tiling(hr_ctx* ctx, int* ty, int* tz, int* version) {
+  if (!ctx) return 1;
+  if (ty) *ty = ctx->g.ty;
+  if (tz) *tz = ctx->g.tz;
+  if (version) *version = ctx->version;
+  return 0;
+}
+
 int hr_set_graph(hr_ctx* ctx, int steps) {
   if (!ctx) return 1;
   if (steps < 0 || (steps & 1) || steps > 4096) return fail(ctx, 1, "steps per graph must be 0 or an even count <= 4096");

@@ -388,6 +388,12 @@ module hirensi
       integer(c_int), value :: ty, tz, version
     end function hr_set_tiling
 
+    integer(c_int) function hr_get_tiling(ctx, ty, tz, version) bind(C, name="hr_get_tiling")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: ty, tz, version
+    end function hr_get_tiling
+
     integer(c_int) function hr_set_profiling(ctx, mode) bind(C, name="hr_set_profiling")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

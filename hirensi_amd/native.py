@@ -20,7 +20,7 @@ EXPORTS = (
     "hr_ctx_create", "hr_ctx_destroy", "hr_last_error", "hr_get_sizes", "hr_get_store_eq",
     "hr_set_pml", "hr_set_material", "hr_get_diag_m", "hr_scale", "hr_diagonal_scale",
     "hr_reset_scaling", "hr_apply", "hr_time_function", "hr_forward", "hr_adjoint_gradient",
-    "hr_misfit", "hr_recompute_ratio", "hr_set_tiling", "hr_set_profiling", "hr_get_kernel_stats",
+    "hr_misfit", "hr_recompute_ratio", "hr_set_tiling", "hr_get_tiling", "hr_set_profiling", "hr_get_kernel_stats",
     "hr_forward_shots", "hr_gradient_mass", "hr_regularization", "hr_regularize_gradient",
     "hr_forward_cd", "hr_set_history_budget", "hr_get_history_plan", "hr_revolve_schedule",
     "hr_gradient_accumulate", "hr_set_numbering", "hr_surface_force", "hr_set_energy_interval", "hr_get_energy",
@@ -99,6 +99,7 @@ def load():
     L.hr_recompute_ratio.argtypes = [_vp]
     L.hr_recompute_ratio.restype = C.c_double
     L.hr_set_tiling.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.hr_get_tiling.argtypes = [_vp, _vp, _vp, _vp]
     L.hr_set_profiling.argtypes = [_vp, C.c_int]
     L.hr_get_kernel_stats.argtypes = [_vp, _dp, _dp]
     L.hr_forward_cd.argtypes = [_vp, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp, C.c_int, _dp, _dp]

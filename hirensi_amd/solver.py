@@ -152,6 +152,15 @@ class Context:
     def recompute_ratio(self):
         return self.L.hr_recompute_ratio(self.h)
 
+    def tiling(self):
+        """(tile y-extent, z-chunk length, stage-kernel version) in use."""
+        ty, tz, ver = C.c_int(0), C.c_int(0), C.c_int(0)
+        self._check(self.L.hr_get_tiling(self.h, C.addressof(ty), C.addressof(tz), C.addressof(ver)))
+        return ty.value, tz.value, ver.value
+
+    def version(self):
+        return self.tiling()[2]
+
     def set_numbering(self, ID):
         """Equation numbering of the caller (``hr_set_numbering``): ``ID`` is
         (nnode, 9) in lattice node order, the equation number of each DOF or -1

@@ -267,6 +267,9 @@ int hr_revolve_schedule(int nseg, int nsnap, int max_actions, int* actions, int*
  * the last hr_forward. */
 #define HR_TILING_SERIAL_CLASSES 0x100
 int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version);
+/* The stage-kernel configuration in use: tile y-extent, z-chunk length and
+ * kernel version (no counterpart in the reference; for callers that report it). */
+int hr_get_tiling(hr_ctx* ctx, int* ty, int* tz, int* version);
 
 /* Stage-kernel timing (off by default; on = 1, reset = 2).  One stage launch
  * in seven is timed with HIP events recorded right before and after its
