@@ -23,7 +23,12 @@
 using namespace hr;
 
 namespace {
-constexpr int DEFAULT_TZ = 7;   // element layers per z-chunk (sweep of 3..8 at 64x64x32: 7 first)
+// element layers per z-chunk: fused tile classes (sweep of 3..8 at 64x64x32: 7 first) and
+// split path (chunks from the regular domain's bottom; sweep of 5..26 at 64x64x32, PML 6:
+// 10 first, 9 and 13 3 % and 14 % slower)
+constexpr int DEFAULT_TZ = 7;
+constexpr int DEFAULT_TZ_SPLIT = 10;
+int default_tz(int version) { return version == 4 ? DEFAULT_TZ_SPLIT : DEFAULT_TZ; }
 constexpr int DEFAULT_TY = 3;
 constexpr int DEFAULT_VERSION = 4;   // 4 = split stage path, 3 = fused tile classes
 
@@ -569,10 +574,15 @@ int setup_tiles(hr_ctx* ctx) {
   g.tiles[0] = (g.nex + g.tx - 1) / g.tx;
   g.tiles[1] = (g.ney + g.ty - 1) / g.ty;
   // z-chunks of tz element layers (the v3 kernel reads the boundaries, so chunks could differ;
-  // aligning one with the bottom PML interface measured slower: PML-only slabs cost as much)
+  // for the fused tile classes, aligning one with the bottom PML interface measured slower:
+  // PML-only slabs cost as much).  The split path's tile kernel covers regular-domain
+  // elements only: its chunks start at the regular domain's bottom layer (the layers
+  // below are one chunk, run by the fused kernels of single-operator applies only)
   std::vector<int>& zb = ctx->zb;
   zb.assign(1, 0);
-  for (int z = g.tz; z < g.nez; z += g.tz) zb.push_back(z);
+  const int z0 = (ctx->version == 4 && g.rlo[2] > 0 && g.rlo[2] < g.nez) ? g.rlo[2] : 0;
+  if (z0 > 0) zb.push_back(z0);
+  for (int z = z0 + g.tz; z < g.nez; z += g.tz) zb.push_back(z);
   zb.push_back(g.nez);
   if ((int)zb.size() - 1 > HR_MAX_ZCHUNKS) return fail(ctx, 1, "too many z-chunks (%d); raise tz", (int)zb.size() - 1);
   g.tiles[2] = (int)zb.size() - 1;
@@ -866,7 +876,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
   g.top_rule = d->top_stress_bc ? 1 : 0;
   g.tx = 4;
   g.ty = DEFAULT_TY;
-  g.tz = DEFAULT_TZ;
+  g.tz = default_tz(ctx->version);
   // affine element: DJ = diag(h/2); DETJ and DJI as in ELEMENT_K_3D
   const double j11 = d->h[0] * 0.5, j22 = d->h[1] * 0.5, j33 = d->h[2] * 0.5;
   const double detj = j11 * j22 * j33 + 0.0 * 0.0 * 0.0 + 0.0 * 0.0 * 0.0 - 0.0 * j22 * 0.0 - 0.0 * 0.0 * j11 - j33 * 0.0 * 0.0;
@@ -915,6 +925,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
     ctx->split_ok = true;
   } else {
     ctx->version = 3;
+    g.tz = default_tz(ctx->version);
   }
   if (int rc = setup_tiles(ctx)) return rc;
   if (int rc = upload(ctx, &ctx->d_eq_slot, ctx->eq_slot)) return rc;
@@ -1210,7 +1221,7 @@ int hr_set_tiling(hr_ctx* ctx, int ty, int tz, int version) {
   if (version) ctx->version = version;
   g.tx = 4;
   g.ty = t;
-  g.tz = tz ? tz : DEFAULT_TZ;
+  g.tz = tz ? tz : default_tz(ctx->version);
   if (int rc = ensure_split(ctx)) return rc;
   return setup_tiles(ctx);
 }
@@ -1939,7 +1950,7 @@ int hr_autotune(hr_ctx* ctx, int nshot, int steps, int* tz_best, double* ms_per_
   int bt = tz0;
   int status = 0;
   // z-chunk candidates: element layers per chunk of the stage-kernel tiles
-  for (int tz : {3, 4, 5, 6, 7, 8, 10, 12}) {
+  for (int tz : {3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14}) {
     if (tz > ctx->g.nez) continue;
     ctx->g.tz = tz;
     if (setup_tiles(ctx) != 0) continue;      // too many chunks for this mesh

@@ -204,6 +204,10 @@ __global__ void __launch_bounds__(GNT) gradient_tile_kernel(Geo g, int bi0, int 
   for (int d = 0; d < 4; ++d) fetch(bi, bj, kc0 - 2 + d, qu[d], qp[d]);
   const int wx = GH + tx, wy = GH + ty;
   for (int k = kc0; k < kc1; ++k) {
+    // this plane's accumulators, read with the plane's inputs
+    const int r = bi + bnx * (bj + bny * k);
+    double gl0 = 0.0, gm0 = 0.0;
+    if (own) { gl0 = glam[r]; gm0 = gmu[r]; }
     fetch(bi, bj, k + 2, qu[4], qp[4]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) { sU[c][wy][wx] = qu[2][c]; sP[c][wy][wx] = qp[2][c]; }
@@ -277,9 +281,8 @@ __global__ void __launch_bounds__(GNT) gradient_tile_kernel(Geo g, int bi0, int 
             sl += sL * W;
             sm += sM * W;
           }
-      const int r = bi + bnx * (bj + bny * k);
-      glam[r] += sl * dt;
-      gmu[r] += sm * dt;
+      glam[r] = gl0 + sl * dt;
+      gmu[r] = gm0 + sm * dt;
     }
     __syncthreads();
 #pragma unroll

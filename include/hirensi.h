@@ -257,7 +257,8 @@ int hr_get_history_plan(hr_ctx* ctx, int nstep, int nshot, int history, int segm
 int hr_revolve_schedule(int nseg, int nsnap, int max_actions, int* actions, int* nactions, long long* advances);
 
 /* Stage-kernel configuration: tile of 4 x ty elements in x-y (ty = 3), tz
- * elements per z-chunk, kernel version (0 keeps it): 4 (the default) splits
+ * elements per z-chunk (0: 10 for version 4, whose chunks start at the regular
+ * domain's bottom layer; 7 for version 3), kernel version (0 keeps it): 4 (the default) splits
  * each RK4 stage into a nodal pass over the PML nodes, the tiled
  * regular-domain kernel and a nodal pull over the PML nodes; 3 runs the fused
  * tile classes (mixed, regular-domain, PML) for every stage.  Single-operator
@@ -283,7 +284,7 @@ int hr_get_kernel_stats(hr_ctx* ctx, long long* launches, double* ms);
 
 /* Autotuner of the stage-kernel tiling: times `steps` RK4 steps of nshot shots
  * (zero state, no sources: the same launches and traffic) for each z-chunk
- * length of the tiles (3..12 element layers) and keeps the fastest; returns it
+ * length of the tiles (3..14 element layers) and keeps the fastest; returns it
  * and its time per step (ms).  Invalidates the history of the last forward.
  * No counterpart in the reference (a PETSc AIJ product has no tiling). */
 int hr_autotune(hr_ctx* ctx, int nshot, int steps, int* tz_best, double* ms_per_step);
