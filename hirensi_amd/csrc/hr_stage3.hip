@@ -45,6 +45,7 @@ namespace hr {
 namespace {
 
 constexpr int NT3 = 256;
+constexpr int SRC_LDS = 64;   // source entries of a tile staged in LDS (more: read from memory)
 
 __device__ __forceinline__ void sto(double* p, unsigned off, double v) {
   *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + off) = v;
@@ -234,6 +235,8 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
   __shared__ double sSig[HAS_RD ? NEW : 1][6][27];                            // regular domain W * sigma
   __shared__ double sWq[27];
   __shared__ double sDM[9];
+  __shared__ long long sSrcNode[SRC_LDS];
+  __shared__ int sSrcComp[SRC_LDS];
 
   const Geo3& g = A.g;
   const int tid = threadIdx.x;
@@ -283,6 +286,17 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
   const double* __restrict__ tab = A.tab;
   const double* __restrict__ nodet = A.node;
 
+  // the tile's source entries (lattice node, component) in LDS, so that the
+  // finish of every owned node scans them without global loads (the other
+  // fields are read for matching entries only)
+  const int nsrc = src_j1 - src_j0;
+  const bool src_lds = nsrc <= SRC_LDS;
+  if (src_lds) {
+    for (int t = tid; t < nsrc; t += NT3) {
+      sSrcNode[t] = A.src.node[src_j0 + t];
+      sSrcComp[t] = A.src.comp[src_j0 + t];
+    }
+  }
   if (tid < 27) sWq[tid] = tab[tid];
   if (tid < 9) {
     const int a = tid / 3, c = tid - 3 * a;   // DM(a, c) = l_c'(xi_a)
@@ -808,34 +822,39 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
           for (int m = 0; m < 9; ++m) out[m] = raw[m];
         } else {
           const double minv_u = minvu_o, minv_s = minvs_o;
-#pragma unroll
-          for (int m = 0; m < 9; ++m) {
-            double src = 0.0;
-            if (m < 3 && src_j1 > src_j0 && ua) {
-              const Src3* sp = &A.src;
-              for (int jj = src_j0; jj < src_j1; ++jj) {
-                if (sp->node[jj] == (long long)n && sp->comp[jj] == m) {
-                  const double mm = sp->m[jj];
-                  double val;
-                  if (A.src_mode == 1) {
-                    const double f = sp->ftab[((size_t)shot * sp->nstep + (A.step + sdyn)) * 3 + A.fcol];
-                    val = f * (sp->val[jj] * mm);
-                  } else {
-                    const size_t nr = (size_t)sp->nrec;
-                    const size_t base = (size_t)shot * ((size_t)sp->nstep + 1) * nr + ((size_t)jj - (size_t)shot * nr);
-                    const size_t ra = (size_t)(A.ra + sdyn);
-                    const double va = sp->tr[base + ra * nr] - sp->obs[base + ra * nr];
-                    val = A.ca * (va * mm);
-                    if (A.rb_on) {
-                      const size_t rb = (size_t)(A.rb + sdyn);
-                      const double vb = sp->tr[base + rb * nr] - sp->obs[base + rb * nr];
-                      val = val + A.cb * (vb * mm);
-                    }
-                  }
-                  src += val;
+          // this node's source terms per component, the entries in list order
+          double srcv[3] = {0.0, 0.0, 0.0};
+          if (src_j1 > src_j0 && ua) {
+            const Src3* sp = &A.src;
+            for (int jj = src_j0; jj < src_j1; ++jj) {
+              const long long nd = src_lds ? sSrcNode[jj - src_j0] : sp->node[jj];
+              if (nd != (long long)n) continue;
+              const int cpt = src_lds ? sSrcComp[jj - src_j0] : sp->comp[jj];
+              const double mm = sp->m[jj];
+              double val;
+              if (A.src_mode == 1) {
+                const double f = sp->ftab[((size_t)shot * sp->nstep + (A.step + sdyn)) * 3 + A.fcol];
+                val = f * (sp->val[jj] * mm);
+              } else {
+                const size_t nr = (size_t)sp->nrec;
+                const size_t base = (size_t)shot * ((size_t)sp->nstep + 1) * nr + ((size_t)jj - (size_t)shot * nr);
+                const size_t ra = (size_t)(A.ra + sdyn);
+                const double va = sp->tr[base + ra * nr] - sp->obs[base + ra * nr];
+                val = A.ca * (va * mm);
+                if (A.rb_on) {
+                  const size_t rb = (size_t)(A.rb + sdyn);
+                  const double vb = sp->tr[base + rb * nr] - sp->obs[base + rb * nr];
+                  val = val + A.cb * (vb * mm);
                 }
               }
+              if (cpt == 0) srcv[0] += val;
+              else if (cpt == 1) srcv[1] += val;
+              else srcv[2] += val;
             }
+          }
+#pragma unroll
+          for (int m = 0; m < 9; ++m) {
+            const double src = m < 3 ? srcv[m] : 0.0;
             const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
             const double gout = src - minv * raw[m];
             if constexpr (!UPD) {
