@@ -157,7 +157,8 @@ struct hr_ctx {
   // and interface sums of split_shots shots, source entries by pull block
   std::vector<std::pair<long long, int>> pnode_pos;   // (lattice index, position in lattice order), by lattice index
   bool split_ok = false;                      // the lattice fits the split path
-  int pnblk = 0;                              // SPLIT_NT-node blocks of the PML nodes
+  int pnblk = 0;                              // SPLIT_NT-node blocks of the PML nodes (nodal pull)
+  int pnblk_nodal = 0;                        // SPLIT_OWN-node blocks of the PML nodes (nodal pass)
   int* d_tl_split = nullptr;
   int ntl_split = 0;
   double* d_D = nullptr;
@@ -493,6 +494,7 @@ int launch_stage(hr_ctx* ctx, StageArgs& A, const Src3Spec* sp = nullptr) {
       Stage3Args Ball = B;
       Ball.pb0 = 0;
       Ball.pnb = ctx->pnblk;
+      Ball.pnbn = ctx->pnblk_nodal;
       if (ctx->concurrent) {
         // the nodal pass on a side stream, the deferred per-step kernels on the
         // other, the regular-domain kernel (the longer one) on the main stream,
@@ -629,12 +631,12 @@ int check_ready(hr_ctx* ctx) {
 }
 
 // state buffers for nshot shots (shot-major); contents are not kept on growth
-// flux records and interface sums of the split path for shot_cap shots; the
-// records of regular-domain-only nodes are never written and stay zero (the
-// pull reads them with zero weights)
+// flux records (y- and z-lines) and interface sums of the split path for
+// shot_cap shots; the records of regular-domain-only nodes are never written and
+// stay zero (the pull reads them)
 int ensure_split(hr_ctx* ctx) {
   if (ctx->version != 4 || ctx->split_shots >= ctx->shot_cap) return 0;
-  const size_t nd = 18 * (size_t)ctx->g.NN * ctx->shot_cap, ne = 3 * (size_t)ctx->g.NN * ctx->shot_cap;
+  const size_t nd = 12 * (size_t)ctx->g.NN * ctx->shot_cap, ne = 3 * (size_t)ctx->g.NN * ctx->shot_cap;
   if (int rc = dalloc(ctx, &ctx->d_D, nd)) return rc;
   if (int rc = dalloc(ctx, &ctx->d_E, ne)) return rc;
   HIPC(hipMemsetAsync(ctx->d_D, 0, nd * sizeof(double), ctx->stream));
@@ -910,7 +912,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
   HIPC(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
   // PML nodes of the split stage path; lattices whose flux records pass 32-bit
   // byte offsets use the fused tile classes
-  if (144.0 * (double)g.NN < 4294967296.0) {
+  if (96.0 * (double)g.NN < 4294967296.0) {
     // the nodal kernels walk the PML nodes in lattice order (sp_pml_node; measured
     // 20 % faster per gradient than with the interface nodes last); their
     // positions in that order, for the pull's per-block source lists
@@ -922,6 +924,7 @@ int hr_ctx_create(const hr_mesh_desc* d, hr_ctx** out) {
     int gl[3] = {g.rlo[0], g.rlo[1], g.rlo[2]}, gh[2] = {g.rhi[0], g.rhi[1]};
     if ((unsigned)pos != sp_pml_count(g.NX, g.NY, g.NZ, gl, gh)) return fail(ctx, 3, "PML node count mismatch");
     ctx->pnblk = (pos + SPLIT_NT - 1) / SPLIT_NT;
+    ctx->pnblk_nodal = (pos + SPLIT_OWN - 1) / SPLIT_OWN;
     ctx->split_ok = true;
   } else {
     ctx->version = 3;

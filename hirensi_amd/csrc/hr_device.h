@@ -149,11 +149,13 @@ struct Stage3Args {
   int nshot;                       // shots in the launch; state vectors are shot-major
   long long shot_stride;           // elements per shot of a state vector (9 NN)
   double hh, h, h6;                // 0.5 dt, dt, dt / 6 (signed); stage 5: dt^2, dt, dt^3
-  // split stage path (RK4 stages): a nodal launch covers the SPLIT_NT-node
-  // blocks [pb0, pb0 + pnb) of the PML nodes in lattice order; the flux records
-  // D (18 NN doubles per shot) and the regular-domain K sums of interface nodes
-  // E (3 NN doubles per shot)
+  // split stage path (RK4 stages): the nodal pull covers the SPLIT_NT-node
+  // blocks [pb0, pb0 + pnb) of the PML nodes in lattice order, the nodal pass its
+  // pnbn blocks of SPLIT_OWN nodes; the y- and z-line flux records D (12 NN
+  // doubles per shot) and the regular-domain K sums of interface nodes E (3 NN
+  // doubles per shot)
   int pb0, pnb;
+  int pnbn;                        // blocks of the nodal pass (SPLIT_OWN list entries each)
   double* D;
   double* E;
 };
@@ -165,7 +167,10 @@ hipError_t launch_stage3(const Stage3Args& A, int stage, int transpose, int tile
 // records, node-local terms, stage-4 x1 x2), the regular-domain kernel over the
 // tiles with regular-domain elements (finishes regular-domain-only nodes, K
 // contribution of interface nodes into E) and the nodal pull over the PML nodes
-constexpr int SPLIT_NT = 256;      // nodes (threads) per block of the nodal kernels
+constexpr int SPLIT_NT = 256;
+// list entries a nodal-pass block owns: it also computes two entries on each
+// side, for the x flux records of its own entries' x-lines
+constexpr int SPLIT_OWN = SPLIT_NT - 4;      // nodes (threads) per block of the nodal kernels
 hipError_t launch_split_nodal(const Stage3Args& A, int stage, int transpose, hipStream_t s);
 hipError_t launch_split_rd(const Stage3Args& A, int stage, int transpose, const int* tiles, int ntiles, hipStream_t s);
 hipError_t launch_split_pull(const Stage3Args& A, int stage, int transpose, hipStream_t s);

@@ -896,8 +896,9 @@ __global__ void __launch_bounds__(NT3, HAS_RD ? (HAS_PML ? 3 : 5) : (S >= 1 && S
 }
 
 // ------------------------------------------------------------ split stage path
-// Blocks of SPLIT_NT consecutive PML nodes in lattice order (sp_pml_node), one
-// node per thread.  XCD x = blockIdx % 8 takes the
+// Blocks of consecutive PML nodes in lattice order (sp_pml_node), one node per
+// thread (nodal pull: SPLIT_NT nodes; nodal pass: SPLIT_OWN nodes and two on each
+// side).  XCD x = blockIdx % 8 takes the
 // logical blocks [x C, (x + 1) C) of the launch, so that each XCD's L2 holds a
 // contiguous run of the list (the neighbouring rows and planes of its nodes); a
 // launch has 8 C blocks and starts at logical block A.pb0.
@@ -910,26 +911,40 @@ __device__ __forceinline__ void st2(double* p, unsigned off, double a, double b)
   *reinterpret_cast<double2*>(reinterpret_cast<char*>(p) + off) = make_double2(a, b);
 }
 
-// Flux records D of the split path, per shot 18 NN doubles: three arrays (ax =
-// x, y, z) of six doubles per lattice node, (G_{x,ax}, G_{y,ax}, G_{z,ax},
+// Flux records D of the split path, per shot 12 NN doubles: two arrays (ax =
+// y, z) of six doubles per lattice node, (G_{x,ax}, G_{y,ax}, G_{z,ax},
 // w_{ax,x}, w_{ax,y}, w_{ax,z}) with G_{i,ax} the nodal PML flux and w_{ax,i}
 // the F-weighted inputs: a line along ax reads three 16-byte pairs per node.
-__device__ __forceinline__ unsigned d_rec(int ax, unsigned NN, unsigned n) { return 48u * ((unsigned)ax * NN + n); }
+// The x-lines are summed by the nodal pass itself, from LDS.
+__device__ __forceinline__ unsigned d_rec(int ax, unsigned NN, unsigned n) {
+  return 48u * ((unsigned)(ax - 1) * NN + n);
+}
 
 // Node-local part of RK4 stage S at one PML node: its stage inputs from the raw
-// inputs (the fused kernel's PML window load, once per node), the flux records,
-// and into the stage output the node-local terms: stages 1-3 -M^-1 (diagonal
-// terms), stage 4 the carry x3 + dt/6 (g1 + 2 g2 + 2 g3) minus dt/6 M^-1
-// (diagonal terms), with the x1, x2 updates written here.
+// inputs (the fused kernel's PML window load, once per node), the flux records
+// of its y- and z-lines, and into the stage output the node-local terms: stages
+// 1-3 -M^-1 (diagonal terms + x-line sums), stage 4 the carry x3 + dt/6 (g1 +
+// 2 g2 + 2 g3) minus dt/6 M^-1 (diagonal terms + x-line sums), with the x1, x2
+// updates written here.  The x-line sums read the x records of the block's
+// entries from LDS: consecutive list entries are x-neighbours.
 template <int S, bool TR>
 __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(const Stage3Args A) {
   constexpr bool NEG = TR;
   constexpr int NIN = 2 + S;
+  __shared__ double sXr[6][SPLIT_NT];     // x flux records of the block's entries
+  __shared__ unsigned sXn[SPLIT_NT];      // their lattice nodes (~0: no entry)
   const Geo3& g = A.g;
-  const int lb = split_block(A.pnb);
-  if (lb >= A.pnb) return;
-  int i, j, k;
-  if (!sp_pml_node((unsigned)((A.pb0 + lb) * SPLIT_NT) + threadIdx.x, g.NX, g.NY, g.NZ, g.rlo, g.rhi, i, j, k)) return;
+  const int lb = split_block(A.pnbn);
+  if (lb >= A.pnbn) return;
+  const int tid = threadIdx.x;
+  // list entry of this thread: the block owns [lb SPLIT_OWN, (lb + 1) SPLIT_OWN) and
+  // computes two more on each side for their x records; a thread without an
+  // entry runs node 0 (in bounds) and stores nothing
+  const long long t = (long long)lb * SPLIT_OWN - 2 + tid;
+  int i = 0, j = 0, k = 0;
+  const bool valid = t >= 0 && sp_pml_node((unsigned)t, g.NX, g.NY, g.NZ, g.rlo, g.rhi, i, j, k);
+  if (!valid) i = j = k = 0;
+  const bool own = valid && tid >= 2 && tid < SPLIT_NT - 2;
   const int shot = blockIdx.y;
   const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
   const long long so = (long long)shot * A.shot_stride;
@@ -987,7 +1002,7 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
     double xn0[3], xn1[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) rk4_update12(XU[c], A.hh, A.h, A.h6, xn0[c], xn1[c], c3[c]);
-    if (ua) {
+    if (own && ua) {
       st3(x1n, offu, xn0[0], xn0[1], xn0[2]);
       st3(x2n, offu, xn1[0], xn1[1], xn1[2]);
     }
@@ -1030,14 +1045,14 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
       double xn0[3], xn1[3];
 #pragma unroll
       for (int tt = 0; tt < 3; ++tt) rk4_update12(XS[tt], A.hh, A.h, A.h6, xn0[tt], xn1[tt], c3[3 + 3 * hf + tt]);
-      const int act = (sm >> (3 * hf)) & 7;
+      const int act = own ? (sm >> (3 * hf)) & 7 : 0;
       if (act) {
         st3m(x1n, offs + 24u * hf, act, xn0[0], xn0[1], xn0[2]);
         st3m(x2n, offs + 24u * hf, act, xn1[0], xn1[1], xn1[2]);
       }
     }
   }
-  // node-local terms of the stage output
+  // node-local terms of the stage output, without the x-line sums
   const double sc = S == 4 ? A.h6 : 1.0;
   double pre[9];
 #pragma unroll
@@ -1045,21 +1060,76 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
     const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
     pre[m] = c3[m] - sc * (minv * dg[m]);
   }
+  // x records to LDS; y and z records to memory
+  sXn[tid] = valid ? n : ~0u;
+  sXr[0][tid] = Gv[0]; sXr[1][tid] = Gv[3]; sXr[2][tid] = Gv[6];
+  sXr[3][tid] = wv[0]; sXr[4][tid] = wv[1]; sXr[5][tid] = wv[2];
+  // flux records: (G_{x,ax}, G_{y,ax}, G_{z,ax}, w_{ax,x}, w_{ax,y}, w_{ax,z}), Gv[3 i + ax], wv[3 ax + i]
+  double* const Dp = A.D + (long long)shot * 12 * g.NN;
+  if (own) {
+#pragma unroll
+    for (int a = 1; a < 3; ++a) {
+      const unsigned o = d_rec(a, NN, n);
+      st2(Dp, o, Gv[a], Gv[3 + a]);
+      st2(Dp, o + 16u, Gv[6 + a], wv[3 * a]);
+      st2(Dp, o + 32u, wv[3 * a + 1], wv[3 * a + 2]);
+    }
+  }
+  __syncthreads();
+  if (!own) return;
+  // the x-line of the pull: side-scaled stencil over the x-neighbours' records.
+  // The neighbour at lattice offset di (|di| <= 2) is at most |di| entries away
+  // in the list (no PML node between, or one); one that is not a PML node (a
+  // regular-domain node) has no record, and offsets outside the lattice have zero
+  // coefficients.
+  const SpAxis AX = sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), AY = sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
+               AZ = sp_axis(k, g.nez, g.rlo[2], g.nez);
+  double cs0, cs1, cu[5];
+  sp_side_scales(AX, AY, AZ, cs0, cs1);
+  sp_stencil(AX.odd, cs0, cs1, cu);
+  double sux[3] = {0.0, 0.0, 0.0}, sdx[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int d = 0; d < 5; ++d) {
+    const int di = d - 2;
+    int slot = -1;
+    if (di == 0) {
+      slot = tid;
+    } else if (i + di >= 0 && i + di < g.NX) {
+      const unsigned ne = (unsigned)((int)n + di);
+      const int near = tid + (di > 0 ? 1 : -1);
+      if (di == 2 || di == -2) slot = sXn[tid + di] == ne ? tid + di : (sXn[near] == ne ? near : -1);
+      else slot = sXn[near] == ne ? near : -1;
+    }
+    // u rows with cu, s rows with every offset but 0 negated
+    const double cw = d == 2 ? cu[d] : -cu[d];
+    if (slot >= 0) {
+      sux[0] += cu[d] * sXr[0][slot]; sux[1] += cu[d] * sXr[1][slot]; sux[2] += cu[d] * sXr[2][slot];
+      sdx[0] += cw * sXr[3][slot]; sdx[1] += cw * sXr[4][slot]; sdx[2] += cw * sXr[5][slot];
+    }
+  }
+  // its terms of the sums, taken into the node-local terms
+  double rx[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) rx[c] = g.detj * (g.dxi[0] * sux[c]);
+  const double cx = g.detj * g.dxi[0];
+  const double ddx0 = cx * sdx[0], ddx1 = cx * sdx[1], ddx2 = cx * sdx[2];
+  if constexpr (!TR) {
+    rx[3] = ddx0; rx[4] = 0.0; rx[5] = 0.0; rx[6] = ddx1; rx[7] = ddx2; rx[8] = 0.0;
+  } else {
+    rx[3] = -(l2m * ddx0); rx[4] = -(lam * ddx0); rx[5] = -(lam * ddx0);
+    rx[6] = -(m2 * ddx1); rx[7] = -(m2 * ddx2); rx[8] = 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    const double minv = m < 3 ? minv_u : (m < 6 ? minv_s : 0.5 * minv_s);
+    pre[m] = pre[m] - sc * (minv * rx[m]);
+  }
   double* const dst = (S == 4 ? A.xnew[2] : A.out) + so;
   if (ua) st3(dst, offu, pre[0], pre[1], pre[2]);
   if (sm == 0x3f) st6(dst, offs, &pre[3]);
   else {
     if (sm & 7) st3m(dst, offs, sm & 7, pre[3], pre[4], pre[5]);
     if (sm & 56) st3m(dst, offs + 24u, (sm >> 3) & 7, pre[6], pre[7], pre[8]);
-  }
-  // flux records: (G_{x,ax}, G_{y,ax}, G_{z,ax}, w_{ax,x}, w_{ax,y}, w_{ax,z}), Gv[3 i + ax], wv[3 ax + i]
-  double* const Dp = A.D + (long long)shot * 18 * g.NN;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const unsigned o = d_rec(a, NN, n);
-    st2(Dp, o, Gv[a], Gv[3 + a]);
-    st2(Dp, o + 16u, Gv[6 + a], wv[3 * a]);
-    st2(Dp, o + 32u, wv[3 * a + 1], wv[3 * a + 2]);
   }
 }
 
@@ -1101,15 +1171,16 @@ __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args 
   const SpAxis ax3[3] = {sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
                          sp_axis(k, g.nez, g.rlo[2], g.nez)};
   const bool iface = sp_iface(ax3[0], ax3[1], ax3[2]);
-  const double* const Dp = A.D + (long long)shot * 18 * g.NN;
+  const double* const Dp = A.D + (long long)shot * 12 * g.NN;
   const int cc[3] = {i, j, k};
   const int nax[3] = {g.NX, g.NY, g.NZ};
   const int str[3] = {1, g.NX, g.NX * g.NY};
-  // per axis: su = sum_d cu[d] G_{.,ax}(n + (d - 2) str), sd the same with the
-  // s-row stencil (every offset but 0 negated) on w_{ax,.}
+  // y- and z-lines (the x-line terms are in the node-local terms): per axis
+  // su = sum_d cu[d] G_{.,ax}(n + (d - 2) str), sd the same with the s-row
+  // stencil (every offset but 0 negated) on w_{ax,.}
   double su[3][3], sd[3][3];
 #pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
+  for (int ax = 1; ax < 3; ++ax) {
     double s0, s1, cu[5];
     sp_side_scales(ax3[ax], ax3[ax == 0 ? 1 : 0], ax3[ax == 2 ? 1 : 2], s0, s1);
     sp_stencil(ax3[ax].odd, s0, s1, cu);
@@ -1117,9 +1188,9 @@ __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args 
     for (int c = 0; c < 3; ++c) { su[ax][c] = 0.0; sd[ax][c] = 0.0; }
 #pragma unroll
     for (int d = 0; d < 5; ++d) {
-      // odd coordinates have no +-2 terms: skipped in y and z, where a row of
-      // consecutive list entries shares the coordinate
-      if (ax > 0 && (d == 0 || d == 4) && ax3[ax].odd) continue;
+      // odd coordinates have no +-2 terms: skipped (a row of consecutive list
+      // entries shares the y and z coordinates)
+      if ((d == 0 || d == 4) && ax3[ax].odd) continue;
       // offsets outside the lattice have zero coefficients: read the node itself
       const int p = cc[ax] + d - 2;
       const unsigned m = (p >= 0 && p < nax[ax]) ? (unsigned)((int)n + (d - 2) * str[ax]) : n;
@@ -1135,16 +1206,15 @@ __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args 
   }
   double raw[9];
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
-    raw[c] = g.detj * (((0.0 + g.dxi[0] * su[0][c]) + g.dxi[1] * su[1][c]) + g.dxi[2] * su[2][c]);
+  for (int c = 0; c < 3; ++c) raw[c] = g.detj * ((0.0 + g.dxi[1] * su[1][c]) + g.dxi[2] * su[2][c]);
 #pragma unroll
   for (int c = 0; c < 3; ++c) raw[c] += iface ? e3[c] : 0.0;
   double dd[3][3];
 #pragma unroll
-  for (int ax = 0; ax < 3; ++ax)
+  for (int ax = 1; ax < 3; ++ax)
 #pragma unroll
     for (int c = 0; c < 3; ++c) dd[ax][c] = (g.detj * g.dxi[ax]) * sd[ax][c];
-  const double aE[6] = {dd[0][0], dd[1][1], dd[2][2], dd[1][0] + dd[0][1], dd[2][0] + dd[0][2], dd[2][1] + dd[1][2]};
+  const double aE[6] = {0.0, dd[1][1], dd[2][2], dd[1][0], dd[2][0], dd[2][1] + dd[1][2]};
   if constexpr (!TR) {
 #pragma unroll
     for (int m = 0; m < 6; ++m) raw[3 + m] = aE[m];
@@ -1217,8 +1287,9 @@ hipError_t launch_split(const Stage3Args& A, int part, const int* tiles, int ntl
     const int nb = 8 * ((ntl * A.nshot + 7) / 8);
     stage_kernel3<4, 3, true, false, S, TR, true><<<nb, NT3, 0, s>>>(A, tiles, ntl);
   } else {
-    if (A.pnb <= 0) return hipSuccess;
-    const dim3 grid(8 * ((A.pnb + 7) / 8), A.nshot);
+    const int nb = part == 0 ? A.pnbn : A.pnb;
+    if (nb <= 0) return hipSuccess;
+    const dim3 grid(8 * ((nb + 7) / 8), A.nshot);
     if (part == 0) pml_nodal_kernel<S, TR><<<grid, SPLIT_NT, 0, s>>>(A);
     else pml_pull_kernel<S, TR><<<grid, SPLIT_NT, 0, s>>>(A);
   }
