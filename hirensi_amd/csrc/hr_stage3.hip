@@ -1139,13 +1139,32 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
 // sources, then out = (node-local terms) + sc (src - M^-1 sums), sc = 1 or dt/6.
 template <int S, bool TR>
 __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args A) {
+  __shared__ int sPn[SRC_LDS], sPj[SRC_LDS], sPc[SRC_LDS];   // the block's source entries
   const Geo3& g = A.g;
   const int lb = split_block(A.pnb);
   if (lb >= A.pnb) return;
   const int blk = A.pb0 + lb;
+  const int shot = blockIdx.y;
+  // this block's source entries (a uniform range), staged in LDS: lattice node,
+  // entry and component, so that each node scans them without global loads
+  int e0 = 0, e1 = 0;
+  if (A.src_mode) {
+    const int pb = shot * A.src.pnblk + blk;
+    e0 = A.src.pblk_ptr[pb];
+    e1 = A.src.pblk_ptr[pb + 1];
+  }
+  const bool src_lds = e1 - e0 <= SRC_LDS;
+  if (e1 > e0 && src_lds) {
+    for (int t = threadIdx.x; t < e1 - e0; t += SPLIT_NT) {
+      const int jj = A.src.pblk_idx[e0 + t];
+      sPj[t] = jj;
+      sPn[t] = (int)A.src.node[jj];
+      sPc[t] = A.src.comp[jj];
+    }
+    __syncthreads();
+  }
   int i, j, k;
   if (!sp_pml_node((unsigned)(blk * SPLIT_NT) + threadIdx.x, g.NX, g.NY, g.NZ, g.rlo, g.rhi, i, j, k)) return;
-  const int shot = blockIdx.y;
   const unsigned n = (unsigned)i + (unsigned)g.NX * ((unsigned)j + (unsigned)g.NY * (unsigned)k);
   const long long so = (long long)shot * A.shot_stride;
   const unsigned NN = (unsigned)g.NN;
@@ -1230,14 +1249,12 @@ __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args 
   // sources of this node (the entries of this block, a uniform range)
   const bool ua = u_act3(g, i, j, k);
   double src[3] = {0.0, 0.0, 0.0};
-  if (A.src_mode) {
+  if (e1 > e0 && ua) {
     const Src3& sp = A.src;
     const int sdyn = A.step_dev ? *A.step_dev : 0;
-    const int pb = shot * sp.pnblk + blk;
-    const int e0 = sp.pblk_ptr[pb], e1 = sp.pblk_ptr[pb + 1];
     for (int e = e0; e < e1; ++e) {
-      const int jj = sp.pblk_idx[e];
-      if (sp.node[jj] != (long long)n || !ua) continue;
+      const int jj = src_lds ? sPj[e - e0] : sp.pblk_idx[e];
+      if ((src_lds ? (long long)sPn[e - e0] : sp.node[jj]) != (long long)n) continue;
       const double mm = sp.m[jj];
       double val;
       if (A.src_mode == 1) {
@@ -1255,7 +1272,7 @@ __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args 
           val = val + A.cb * (vb * mm);
         }
       }
-      const int cpt = sp.comp[jj];
+      const int cpt = src_lds ? sPc[e - e0] : sp.comp[jj];
       src[0] += cpt == 0 ? val : 0.0;
       src[1] += cpt == 1 ? val : 0.0;
       src[2] += cpt == 2 ? val : 0.0;
