@@ -97,10 +97,10 @@ def stage_bytes(neq, nnode):
 def split_stage_bytes(mesh, transpose=False):
     """Algorithmic bytes of one RK4 stage s of the split stage path (the default):
     the nodal pass over the n_p PML nodes reads the 2+s raw inputs of all nine
-    components and six node-table doubles, writes the 12 flux-record doubles of
+    components and four node-table doubles, writes the 12 flux-record doubles of
     the y- and z-lines (the x-lines are summed in the pass, from LDS) and the
     nine node-local outputs (stage 4: also x1, x2); the nodal pull reads the
-    flux records (each once), the node-local outputs, M^-1, the interface sums
+    flux records (each once), the node-local outputs, M^-1 of the u rows, the interface sums
     (and lambda, mu when transposed) and writes nine outputs; the regular-domain
     kernel reads, per regular-domain-only node, the displacement of the raw inputs
     that enter its stage input (1, 2, 3, 3 vectors; stage 4: all six for the
@@ -112,8 +112,8 @@ def split_stage_bytes(mesh, transpose=False):
     n_r = mesh.nnode - n_p
     b = {}
     for st in (1, 2, 3, 4):
-        k1 = n_p * (9 * (2 + st) + 6 + 12 + 9 + (18 if st == 4 else 0))
-        k3 = n_p * (12 + 9 + 2 + 3 + (2 if transpose else 0) + 9)
+        k1 = n_p * (9 * (2 + st) + 4 + 12 + 9 + (18 if st == 4 else 0))
+        k3 = n_p * (12 + 9 + 1 + 3 + (2 if transpose else 0) + 9)
         nv = 6 if st == 4 else {1: 1, 2: 2, 3: 3}[st]
         k2 = n_r * (3 * nv + 2 + 2 + (9 if st == 4 else 3))
         b[st] = 8.0 * (k1 + k2 + k3)

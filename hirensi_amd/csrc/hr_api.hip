@@ -109,6 +109,8 @@ struct hr_ctx {
   double *d_ax = nullptr, *d_ay = nullptr;
   // regular-domain lumped mass per node (DIAG_M_RD of each displacement DOF)
   double* d_mrd = nullptr;
+  double* d_minvu = nullptr;                  // split path node tables: M^-1 of u rows, rho W over PML elements
+  double* d_rwp = nullptr;
   // regular-domain energy every e_every steps of the forward (0: off)
   int e_every = 0, e_rows = 0;
   double *d_energy = nullptr, *d_epart = nullptr;
@@ -434,6 +436,8 @@ Stage3Args args3(hr_ctx* ctx, const StageArgs& A, const Src3Spec* sp) {
   for (int q = 0; q < 3; ++q) B.xnew[q] = A.xnew[q];
   B.node = ctx->d_node;
   B.tab = ctx->d_tab;
+  B.minvu = ctx->d_minvu;
+  B.rwp = ctx->d_rwp;
   B.set_mask = A.set_mask;
   B.hh = A.hh; B.h = A.h; B.h6 = A.h6;
   if (sp && sp->mode) {
@@ -972,7 +976,7 @@ int hr_ctx_destroy(hr_ctx* ctx) {
   for (double* p : ctx->snap) F(p);
   for (int a = 0; a < 6; ++a) F(ctx->adj_buf[a]);
   F(ctx->d_mis_ptr); F(ctx->d_mis_idx); F(ctx->d_ax); F(ctx->d_ay);
-  F(ctx->d_mrd); F(ctx->d_energy); F(ctx->d_epart); F(ctx->d_uslice);
+  F(ctx->d_mrd); F(ctx->d_minvu); F(ctx->d_rwp); F(ctx->d_energy); F(ctx->d_epart); F(ctx->d_uslice);
   F(ctx->d_src_tile_ptr); F(ctx->d_src_node); F(ctx->d_src_comp); F(ctx->d_src_m); F(ctx->d_src_val);
   F(ctx->d_rec_tile_ptr); F(ctx->d_rec_node); F(ctx->d_rec_comp); F(ctx->d_rec_m); F(ctx->d_rec_perm);
   F(ctx->d_obs); F(ctx->d_gl); F(ctx->d_gm); F(ctx->d_part); F(ctx->d_scalar); F(ctx->d_tr_sorted);
@@ -1110,6 +1114,11 @@ int hr_set_material(hr_ctx* ctx, const double* lam, const double* mu, const doub
     if (int rc = upload(ctx, &ctx->d_tab, tb)) return rc;
   }
   if (int rc = upload(ctx, &ctx->d_mrd, mrd)) return rc;
+  // the split path's nodal kernels read M^-1 of the u rows and the PML rho W sum
+  // from arrays of their own (8 bytes a node each); the stress rows' M^-1 and
+  // weight sum follow from the stretch factors and the element count
+  if (int rc = upload(ctx, &ctx->d_minvu, ctx->h_minv_u)) return rc;
+  if (int rc = upload(ctx, &ctx->d_rwp, rwp)) return rc;
   ctx->have_material = true;
   // the history and traces of an earlier forward belong to the old operator
   ctx->f_nstep = 0;

@@ -139,6 +139,8 @@ struct Stage3Args {
   double* xnew[3];                 // stage 4: updated x1, x2, x3; stage 5: u_{n+1}, u_his
   const double* node;              // node table
   const double* tab;               // small table
+  const double* minvu;             // split path: M^-1 of the displacement rows per node (NN)
+  const double* rwp;               // split path: sum of rho W over the PML elements per node (NN)
   Src3 src;                        // source entries, used when src_mode != 0
   int set_mask;                    // apply: bit0 G, bit1 K, bit2 C
   int src_mode;                    // 0 none, 1 forward: f * B * m, 2 adjoint misfit

@@ -69,6 +69,15 @@ constexpr void sp_stencil(int odd, double s0, double s1, double cu[5]) {
   cu[4] = odd ? 0.0 : s1 / 6.0;
 }
 
+// sum of w(a) w(b) w(c) over the PML elements holding the node (times det J:
+// the node's PML weight sum): the elements holding it are the products of its
+// per-axis sides, the regular-domain ones those of its regular-domain sides
+constexpr double sp_pml_weight(const SpAxis& X, const SpAxis& Y, const SpAxis& Z) {
+  const int n = (X.v[0] + X.v[1]) * (Y.v[0] + Y.v[1]) * (Z.v[0] + Z.v[1]);
+  const int r = (X.r[0] + X.r[1]) * (Y.r[0] + Y.r[1]) * (Z.r[0] + Z.r[1]);
+  return sp_wn(X) * sp_wn(Y) * sp_wn(Z) * (double)(n - r);
+}
+
 // whether a regular-domain element holds the node (an interface node)
 constexpr bool sp_iface(const SpAxis& X, const SpAxis& Y, const SpAxis& Z) {
   return (X.r[0] + X.r[1]) * (Y.r[0] + Y.r[1]) * (Z.r[0] + Z.r[1]) > 0;

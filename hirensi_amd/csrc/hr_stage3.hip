@@ -950,7 +950,7 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
   const long long so = (long long)shot * A.shot_stride;
   const unsigned NN = (unsigned)g.NN;
   const unsigned SB = 8u * (3u * NN + (NN & 1u));
-  const unsigned offu = n * 24u, offs = SB + n * 48u, offq = 16u * NN + n * 32u;
+  const unsigned offu = n * 24u, offs = SB + n * 48u;
   const double* __restrict__ tab = A.tab;
   const double* __restrict__ nodet = A.node;
 
@@ -963,13 +963,17 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
     ld2(ip, offs + 16u, XS6[2][q], XS6[3][q]);
     ld2(ip, offs + 32u, XS6[4][q], XS6[5][q]);
   }
-  double lam, mu, minv_u, minv_s, rwp, wps, ax, bx, ay, by, az, bz;
+  double lam, mu, ax, bx, ay, by, az, bz;
   ld2(nodet, n * 16u, lam, mu);
-  ld2(nodet, offq, minv_u, minv_s);
-  ld2(nodet, offq + 16u, rwp, wps);
+  const double minv_u = A.minvu[n], rwp = A.rwp[n];
   ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * i), ax, bx);
   ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * j), ay, by);
   ld2(tab, 8u * (unsigned)(TAB_STRETCH + 2 * g.NX + 2 * g.NY + 2 * k), az, bz);
+  // the node's sides per axis; its PML weight sum and the stress rows' M^-1
+  const SpAxis AX = sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), AY = sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
+               AZ = sp_axis(k, g.nez, g.rlo[2], g.nez);
+  const double wps = g.detj * sp_pml_weight(AX, AY, AZ);
+  const double minv_s = 1.0 / (((ax * ay) * az) * wps);
   const bool ua = u_act3(g, i, j, k);
   const int sm = smask3(g, i, j, k);
 
@@ -1082,8 +1086,6 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
   // in the list (no PML node between, or one); one that is not a PML node (a
   // regular-domain node) has no record, and offsets outside the lattice have zero
   // coefficients.
-  const SpAxis AX = sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), AY = sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
-               AZ = sp_axis(k, g.nez, g.rlo[2], g.nez);
   double cs0, cs1, cu[5];
   sp_side_scales(AX, AY, AZ, cs0, cs1);
   sp_stencil(AX.odd, cs0, cs1, cu);
@@ -1174,8 +1176,10 @@ __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args 
 
   // node-local terms the nodal pass wrote, and M^-1, issued with the records
   double* const dst = (S == 4 ? A.xnew[2] : A.out) + so;
-  double minv_u, minv_s;
-  ld2(nodet, 16u * NN + n * 32u, minv_u, minv_s);
+  const double minv_u = A.minvu[n];
+  const double* __restrict__ tab = A.tab;
+  const double sax = tab[TAB_STRETCH + 2 * i], say = tab[TAB_STRETCH + 2 * g.NX + 2 * j],
+               saz = tab[TAB_STRETCH + 2 * g.NX + 2 * g.NY + 2 * k];
   double pre[9];
   ld3(dst, offu, pre[0], pre[1], pre[2]);
   ld2(dst, offs, pre[3], pre[4]);
@@ -1190,6 +1194,8 @@ __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args 
   const SpAxis ax3[3] = {sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
                          sp_axis(k, g.nez, g.rlo[2], g.nez)};
   const bool iface = sp_iface(ax3[0], ax3[1], ax3[2]);
+  // M^-1 of the stress rows: 1 / (a W), a the stretch-factor product, W the PML weight sum
+  const double minv_s = 1.0 / (((sax * say) * saz) * (g.detj * sp_pml_weight(ax3[0], ax3[1], ax3[2])));
   const double* const Dp = A.D + (long long)shot * 12 * g.NN;
   const int cc[3] = {i, j, k};
   const int nax[3] = {g.NX, g.NY, g.NZ};

@@ -1,9 +1,10 @@
 """Host check of the nodal PML pull coefficients of the split stage path
-(hirensi_amd/csrc/hr_split.h): the side-scaled per-axis stencils against a
-brute-force sum over the PML elements holding each node, on boxes with
-regular-domain interfaces on every side (Element_Matrices_3D.F90:618-882 gives
-the element terms: W(point) DM(p, l) on the u rows, W(node) DM(l, p) on the s
-rows)."""
+(hirensi_amd/csrc/hr_split.h): the side-scaled per-axis stencils and the PML
+weight sums against brute-force sums over the PML elements holding each node,
+on boxes with regular-domain interfaces on every side (Element_Matrices_3D.F90:618-882
+gives the element terms: W(point) DM(p, l) on the u rows, W(node) DM(l, p) on
+the s rows); the list-free PML node enumeration and the nodal pass's x-neighbour
+rule against a lattice walk."""
 import os
 import shutil
 import subprocess
