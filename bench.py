@@ -101,21 +101,25 @@ def split_stage_bytes(mesh, transpose=False):
     the y- and z-lines (the x-lines are summed in the pass, from LDS) and the
     nine node-local outputs (stage 4: also x1, x2); the nodal pull reads the
     flux records (each once), the node-local outputs, M^-1 of the u rows, the interface sums
-    (and lambda, mu when transposed) and writes nine outputs; the regular-domain
+    of the n_i interface nodes (and lambda, mu when transposed) and writes nine outputs; the regular-domain
     kernel reads, per regular-domain-only node, the displacement of the raw inputs
     that enter its stage input (1, 2, 3, 3 vectors; stage 4: all six for the
-    update), lambda, mu and M^-1, and writes three outputs (stage 4: nine)."""
+    update), lambda, mu and M^-1, and writes three outputs (stage 4: nine), and the
+    interface sums of the interface nodes."""
     lo, hi = mesh.rd_lo, mesh.rd_hi
     nxy = mesh.NX * mesh.NY
     ax, ay = 2 * (hi[0] - lo[0]) - 1, 2 * (hi[1] - lo[1]) - 1
     n_p = (2 * lo[2] + 1) * nxy + (mesh.NZ - 2 * lo[2] - 1) * (nxy - ax * ay)
     n_r = mesh.nnode - n_p
+    # interface nodes: the regular domain's node box less its interior
+    bx, by, bz = ax + 2, ay + 2, mesh.NZ - 2 * lo[2]
+    n_i = bx * by * bz - ax * ay * (bz - 1)
     b = {}
     for st in (1, 2, 3, 4):
         k1 = n_p * (9 * (2 + st) + 4 + 12 + 9 + (18 if st == 4 else 0))
-        k3 = n_p * (12 + 9 + 1 + 3 + (2 if transpose else 0) + 9)
+        k3 = n_p * (12 + 9 + 1 + (2 if transpose else 0) + 9) + 3 * n_i
         nv = 6 if st == 4 else {1: 1, 2: 2, 3: 3}[st]
-        k2 = n_r * (3 * nv + 2 + 2 + (9 if st == 4 else 3))
+        k2 = n_r * (3 * nv + 2 + 2 + (9 if st == 4 else 3)) + 3 * n_i
         b[st] = 8.0 * (k1 + k2 + k3)
     return b
 

@@ -1187,13 +1187,14 @@ __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args 
   ld2(dst, offs + 32u, pre[7], pre[8]);
   double lam = 0.0, mu = 0.0;
   if constexpr (TR) ld2(nodet, n * 16u, lam, mu);
-  // interface sums, read for every node with the first batch and used (by a
-  // select) only where a regular-domain element holds the node
-  double e3[3];
-  ld3(A.E + (long long)shot * 3 * g.NN, offu, e3[0], e3[1], e3[2]);
+  // interface sums, read with the first batch where a regular-domain element
+  // holds the node; other lanes read the shot's first entry (one shared line)
+  // and drop it
   const SpAxis ax3[3] = {sp_axis(i, g.nex, g.rlo[0], g.rhi[0]), sp_axis(j, g.ney, g.rlo[1], g.rhi[1]),
                          sp_axis(k, g.nez, g.rlo[2], g.nez)};
   const bool iface = sp_iface(ax3[0], ax3[1], ax3[2]);
+  double e3[3];
+  ld3(A.E + (long long)shot * 3 * g.NN, iface ? offu : 0u, e3[0], e3[1], e3[2]);
   // M^-1 of the stress rows: 1 / (a W), a the stretch-factor product, W the PML weight sum
   const double minv_s = 1.0 / (((sax * say) * saz) * (g.detj * sp_pml_weight(ax3[0], ax3[1], ax3[2])));
   const double* const Dp = A.D + (long long)shot * 12 * g.NN;
