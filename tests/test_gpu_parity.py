@@ -194,3 +194,43 @@ def test_checkpoint_gradient_equals_stored():
         out.append(S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dmeas))
     for o in out[1:]:
         assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1], out[0][1])
+
+
+@pytest.mark.parametrize("shape,npml", [((5, 5, 4), (2, 2, 2, 2, 1)), ((9, 7, 6), (1, 3, 2, 2, 2))])
+@pytest.mark.parametrize("kcfg", KCFG)
+def test_forward_adjoint_uneven_boxes(shape, npml, kcfg):
+    """A regular domain one element wide (rows of PML nodes on both sides of a
+    one-node gap: the split path's x-line neighbours) and uneven PML, with
+    receivers on the regular domain's surface nodes, its edges (interface nodes:
+    adjoint sources in the tile kernel and in the nodal pull) included, and a
+    second source in the PML."""
+    from tests.helpers import random_model
+    from hirensi_amd.mesh import receiver_dofs
+    mesh = BoxMesh(*shape, npml=npml, top_stress_bc="reference")
+    model = random_model(mesh, seed=6)
+    lo, hi, top = mesh.rd_lo, mesh.rd_hi, mesh.NZ - 1
+    rec_nodes = [mesh.node_index(i, j, top) for j in range(2 * lo[1], 2 * hi[1] + 1)
+                 for i in range(2 * lo[0], 2 * hi[0] + 1, 2 if hi[0] - lo[0] > 2 else 1)]
+    rec_eq = receiver_dofs(mesh, rec_nodes)
+    src_eq = np.array([mesh.dof(mesh.surface_node(0.0, 0.0), 2), mesh.dof(mesh.node_index(1, 1, top), 0)],
+                      dtype=np.int64)
+    src_val = np.array([1.0, 0.5])
+    nstep, dt, fr = 160, 1.0e-4, 60.0
+    print(f"uneven box {shape} npml={npml} kernel={kcfg} neq={mesh.neq}", flush=True)
+    dmeas = _observed(mesh, rec_eq, src_eq, src_val, nstep, dt, fr, seed=9)
+    ctx = S.Context(mesh, model, version=kcfg[0], ty=kcfg[1])
+    ftab = S.time_table(nstep, dt, 2, fr)
+    tr = S.explicit_rk4_petsc_pml_3d(ctx, nstep, dt, ftab, src_eq, src_val, rec_eq, history=S.HR_HISTORY_STORE)
+    gl, gm, mis = S.explicit_rk4_petsc_pml_3d_adjoint_otd(ctx, nstep, dt, rec_eq, dmeas)
+    sysm = O.System(mesh, model)
+    B = np.zeros(mesh.neq)
+    np.add.at(B, src_eq, src_val)
+    U = sysm.forward(B, nstep, dt, lcase=2, fr=fr)
+    P = sysm.adjoint(U, nstep, dt, rec_eq, dmeas)
+    glr, gmr = sysm.control(U, P, nstep, dt)
+    pos = {int(e): r for r, e in enumerate(mesh.store_eq)}
+    rows = np.array([pos[int(e)] for e in rec_eq])
+    assert np.abs(U[:, rows]).max() > 0 and np.abs(glr).max() > 0
+    assert rel_l2(tr, U[:, rows]) < 1e-10
+    assert rel_l2(gl, glr) < 1e-10
+    assert rel_l2(gm, gmr) < 1e-10
