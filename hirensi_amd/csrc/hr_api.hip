@@ -337,6 +337,7 @@ int build_pull_blocks(hr_ctx* ctx, const std::vector<int>& ptr, const std::vecto
   h3.pblk_ptr = *d_ptr;
   h3.pblk_idx = *d_idx;
   h3.pnblk = nblk;
+  h3.ptop = idx.empty() ? 0 : 1;
   return 0;
 }
 

@@ -130,6 +130,7 @@ struct Src3 {                      // device-resident source entries of one solv
   const int* pblk_ptr;
   const int* pblk_idx;
   int pnblk;
+  int ptop;                        // some entry is at a PML node: the pull walks its block runs top-first
 };
 
 struct Stage3Args {

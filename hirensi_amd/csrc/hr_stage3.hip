@@ -906,9 +906,9 @@ __device__ __forceinline__ int split_block(int nblk) {
   const int C = (nblk + 7) / 8;
   return (blockIdx.x & 7) * C + (blockIdx.x >> 3);
 }
-// the same runs, each walked from its end: the nodal pull dispatches the top
-// planes (where the receivers are, and with them the blocks that also add
-// adjoint sources) first instead of last
+// the same runs, each walked from its end: a nodal pull whose sources include
+// PML nodes dispatches the top planes (surface sources and receivers, and with
+// them the blocks that also add source terms) first instead of last
 __device__ __forceinline__ int split_block_rev(int nblk) {
   const int C = (nblk + 7) / 8;
   return (blockIdx.x & 7) * C + (C - 1 - (int)(blockIdx.x >> 3));
@@ -1150,7 +1150,7 @@ template <int S, bool TR>
 __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args A) {
   __shared__ int sPn[SRC_LDS], sPj[SRC_LDS], sPc[SRC_LDS];   // the block's source entries
   const Geo3& g = A.g;
-  const int lb = split_block_rev(A.pnb);
+  const int lb = (A.src_mode && A.src.ptop) ? split_block_rev(A.pnb) : split_block(A.pnb);
   if (lb >= A.pnb) return;
   const int blk = A.pb0 + lb;
   const int shot = blockIdx.y;
