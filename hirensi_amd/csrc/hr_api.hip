@@ -149,12 +149,13 @@ struct hr_ctx {
   hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   int concurrent = 1;
   // Per-step work (receiver traces, history slices, the per-step gradient) that
-  // only reads the state a stage-1 launch reads and does not write: queued on the
-  // main stream after the next stage launch's fork, so that it runs beside the
-  // side classes instead of between stages.  The next launch_stage consumes it.
+  // only reads the state a stage-1 launch reads and does not write: queued beside
+  // the next stage launch (fused classes: on the main stream after the fork; split
+  // path: on the second side stream), so that it runs beside the stage kernels
+  // instead of between stages.  The next launch_stage consumes it.
   std::vector<std::function<int()>> deferred;
-  // split stage path (version 4): PML nodes in lattice order (packed i | j << 10 |
-  // k << 20, and their lattice indices on the host), the tile list of its
+  // split stage path (version 4): the PML nodes' positions in lattice order (host;
+  // the kernels walk that order without a list), the tile list of its
   // regular-domain kernel (the mixed and regular-domain tiles), the flux records
   // and interface sums of split_shots shots, source entries by pull block
   std::vector<std::pair<long long, int>> pnode_pos;   // (lattice index, position in lattice order), by lattice index
