@@ -1143,9 +1143,10 @@ __global__ void __launch_bounds__(SPLIT_NT, S == 4 ? 2 : 3) pml_nodal_kernel(con
 }
 
 // Pull of RK4 stage S at one PML node: the PML elements' terms from the flux
-// records along the three lattice lines through the node (side-scaled 5-point
-// stencils), the regular-domain K sums of an interface node from E, the
-// sources, then out = (node-local terms) + sc (src - M^-1 sums), sc = 1 or dt/6.
+// records along the y- and z-lines through the node (side-scaled 5-point
+// stencils; the x-line terms are in the node-local terms), the regular-domain K
+// sums of an interface node from E, the sources, then out = (node-local terms)
+// + sc (src - M^-1 sums), sc = 1 or dt/6.
 template <int S, bool TR>
 __global__ void __launch_bounds__(SPLIT_NT, 2) pml_pull_kernel(const Stage3Args A) {
   __shared__ int sPn[SRC_LDS], sPj[SRC_LDS], sPc[SRC_LDS];   // the block's source entries
